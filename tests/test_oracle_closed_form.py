@@ -1,0 +1,237 @@
+"""Pin the oracle with closed forms, identities and brute force (CPU only).
+
+These restate the intent of the reference's physics tests
+(tests/test_core.py:11-393, tests/test_measurement_ground_truth.py:79-224,
+tests/test_wigner_and_husimi.py:13-98) with seeded parameters, so that a
+dropped term, a wrong sign or a transposed operand in the oracle fails here
+independently of the golden vectors.
+"""
+
+import numpy as np
+import pytest
+from numpy import pi, sqrt
+
+from oracle.cvdv_oracle import CvdvOracle
+from oracle.separable_np import SeparableStateNP
+
+RNG = np.random.default_rng(1234)
+
+
+def _sim(regs, init, conv="torch"):
+    sep = SeparableStateNP(regs)
+    for meth, r, *a in init:
+        getattr(sep, meth)(r, *a)
+    s = CvdvOracle(regs, convention=conv)
+    s.initStateVector(sep)
+    return s, sep
+
+
+def _sep(regs, init):
+    sep = SeparableStateNP(regs)
+    for meth, r, *a in init:
+        getattr(sep, meth)(r, *a)
+    return sep
+
+
+def test_centered_dft_brute_force():
+    """ftQ2P equals the centred unitary DFT F[k,j] = N^-1/2 exp(-2 pi i (j-h)(k-h)/N)."""
+    for n in (3, 6):
+        N = 1 << n
+        s, _ = _sim([n], [["setCoeffs", 0, RNG.normal(size=N) + 1j * RNG.normal(size=N)]])
+        psi = s.getState()
+        h = (N - 1) / 2
+        j = np.arange(N)
+        F = np.exp(-2j * pi * np.outer(j - h, j - h) / N) / sqrt(N)
+        s.ftQ2P(0)
+        np.testing.assert_allclose(s.getState(), F @ psi, atol=1e-13)
+        s.ftP2Q(0)
+        np.testing.assert_allclose(s.getState(), psi, atol=1e-13)
+
+
+def test_vacuum_equals_coherent_zero():
+    s, _ = _sim([10], [["setFock", 0, 0]])
+    assert abs(s.getFidelity(_sep([10], [["setCoherent", 0, 0]])) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_displacement_on_vacuum_and_composition(seed):
+    rng = np.random.default_rng(seed)
+    a = complex(*(rng.random(2) - 0.5) * 4)
+    b = complex(*(rng.random(2) - 0.5) * 4)
+    s, _ = _sim([10], [["setFock", 0, 0]])
+    s.d(0, a)
+    assert abs(s.getFidelity(_sep([10], [["setCoherent", 0, a]])) - 1) < 1e-10
+    s.d(0, b)
+    assert abs(s.getFidelity(_sep([10], [["setCoherent", 0, a + b]])) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("n", [0, 3, 11, 19])
+def test_fourier_on_fock(n):
+    """FT|n> = (-i)^n |n> up to discretisation (fidelity 1 to 1e-10)."""
+    s, sep = _sim([10], [["setFock", 0, n]])
+    s.ftQ2P(0)
+    ov = np.vdot(sep.register_arrays[0], s.getState())
+    assert abs(abs(ov) ** 2 / np.vdot(sep.register_arrays[0], sep.register_arrays[0]).real ** 1 - 1) < 1e-9
+    np.testing.assert_allclose(ov / abs(ov), (-1j) ** n, atol=1e-8)
+
+
+def test_qubit_gates_and_probabilities():
+    s, _ = _sim([1], [["setZero", 0]])
+    s.h(0, 0)
+    plus = _sep([1], [["setUniform", 0]])
+    assert abs(s.getFidelity(plus) - 1) < 1e-12
+    s.rz(0, 0, pi)
+    minus = _sep([1], [["setCoeffs", 0, [1 / sqrt(2), -1 / sqrt(2)]]])
+    assert abs(s.getFidelity(minus) - 1) < 1e-12
+    th = 0.83
+    s.rz(0, 0, th)
+    s.h(0, 0)
+    p = s.m(0)
+    # H Rz(th) |-> : P(0) = sin^2(th/2), P(1) = cos^2(th/2)
+    np.testing.assert_allclose(p, [np.sin(th / 2) ** 2, np.cos(th / 2) ** 2], atol=1e-12)
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_conditional_displacement_commutator(seed):
+    rng = np.random.default_rng(seed)
+    x, y = (rng.random(2) - 0.5) * 2
+    s, sep = _sim([1, 10], [["setUniform", 0], ["setFock", 1, 0]])
+    s.cd(1, 0, 0, x)
+    s.d(1, 1j * y)
+    s.cd(1, 0, 0, -x)
+    s.d(1, -1j * y)
+    s.rz(0, 0, 4 * x * y)
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("theta", [0.4, -2.2, 3.7, 4.9])
+def test_rotation_of_coherent(theta):
+    s, _ = _sim([10], [["setCoherent", 0, 3.0]])
+    s.r(0, theta)
+    assert abs(s.getFidelity(_sep([10], [["setCoherent", 0, 3.0 * np.exp(-1j * theta)]])) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("alpha,r", [(1.3, 0.4), (-2.1, -0.7)])
+def test_squeeze_conjugates_displacement(alpha, r):
+    s, sep = _sim([10], [["setCoherent", 0, 3j]])
+    s.s(0, -r)
+    s.d(0, alpha)
+    s.s(0, r)
+    s.d(0, -alpha * np.exp(-r))
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+def test_squeezed_vacuum_wavefunction():
+    """S(r)|0> = e^{r/2} pi^{-1/4} sqrt(dx) exp(-x^2 e^{2r}/2) (analysis/squeeze_err.py:29-39)."""
+    r = 0.35
+    s, _ = _sim([10], [["setFock", 0, 0]])
+    s.s(0, r)
+    x = s.getXGrid(0)
+    dx = s.grid_steps[0]
+    exact = np.exp(r / 2) * pi ** -0.25 * sqrt(dx) * np.exp(-0.5 * x ** 2 * np.exp(2 * r))
+    ov = np.vdot(exact, s.getState())
+    assert abs(abs(ov) ** 2 - 1) < 1e-9
+
+
+@pytest.mark.parametrize("nf,theta", [(3, 0.9), (5, -2.7), (4, 4.1)])
+def test_beam_splitter_binomial(nf, theta):
+    s, _ = _sim([10, 10], [["setFock", 0, nf], ["setFock", 1, 0]])
+    s.bs(0, 1, theta)
+    p1, p2 = np.cos(theta / 2) ** 2, np.sin(theta / 2) ** 2
+    th = p2 ** nf
+    for j in range(nf + 1):
+        if j > 0:
+            th *= (nf - j + 1) / j * p1 / p2
+        f = s.getFidelity(_sep([10, 10], [["setFock", 0, j], ["setFock", 1, nf - j]]))
+        assert abs(f - th) < 1e-10
+
+
+@pytest.mark.parametrize("nf,theta", [(0, 0.7), (6, -2.9)])
+def test_conditional_rotation_identity(nf, theta):
+    s, sep = _sim([1, 10], [["setUniform", 0], ["setFock", 1, nf]])
+    s.cr(1, 0, 0, theta)
+    s.rz(0, 0, theta * -(2 * nf + 1))
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("r,alpha", [(0.45, 0.7), (-0.3, -0.5)])
+def test_conditional_squeeze_identity(r, alpha):
+    s, sep = _sim([1, 10], [["setUniform", 0], ["setFock", 1, 0]])
+    s.cs(1, 0, 0, r)
+    s.d(1, alpha)
+    s.cs(1, 0, 0, -r)
+    s.d(1, -alpha * np.cosh(r))
+    s.cd(1, 0, 0, -alpha * np.sinh(r))
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("nf,theta", [(3, 1.1), (4, -3.6)])
+def test_conditional_beam_splitter_identity(nf, theta):
+    s, sep = _sim([1, 10, 10], [["setUniform", 0], ["setFock", 1, nf], ["setFock", 2, 0]])
+    s.cbs(1, 2, 0, 0, theta)
+    s.cp(2, 0, 0)
+    s.bs(1, 2, -theta)
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+def test_wigner_and_husimi_closed_form():
+    a = 1.0 + 0.5j
+    s, _ = _sim([8], [["setCoherent", 0, a]])
+    dx, N = s.grid_steps[0], 256
+    x = (np.arange(N) - (N - 1) / 2) * dx
+    q0, p0 = sqrt(2) * a.real, sqrt(2) * a.imag
+    pw = (np.arange(N) - N / 2) * pi / (N * dx)
+    X, P = np.meshgrid(x, pw)
+    W = s.getWigner(0)
+    np.testing.assert_allclose(W, dx / pi * np.exp(-(X - q0) ** 2 - (P - p0) ** 2), atol=1e-2)
+    # the (N-1) finalisation phase makes W = W_exact * cos(p dx) on the native grid
+    Wex = dx / pi * np.exp(-(X - q0) ** 2 - (P - p0) ** 2) * np.cos(P * dx)
+    assert np.max(np.abs(W - Wex)) < 1e-6
+    ph = (np.arange(N) - N / 2) * 2 * pi / (N * dx)
+    X, P = np.meshgrid(x, ph)
+    Q = s.getHusimiQ(0)
+    np.testing.assert_allclose(Q, np.exp(-0.5 * (X - q0) ** 2 - 0.5 * (P - p0) ** 2) / pi, atol=1e-12)
+    assert np.all(Q >= -1e-14)
+
+
+@pytest.mark.parametrize("alpha", [0.0, 1.0, 1.5 + 1.0j, -0.5 + 2.0j])
+def test_photon_number_coherent(alpha):
+    s, _ = _sim([8], [["setCoherent", 0, alpha]])
+    before = s.getState()
+    assert abs(s.getPhotonNumber(0) - abs(alpha) ** 2) < 1e-2
+    np.testing.assert_allclose(s.getState(), before, atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 5])
+def test_photon_number_fock(n):
+    s, _ = _sim([8], [["setFock", 0, n]])
+    assert abs(s.getPhotonNumber(0) - n) < 1e-2
+
+
+def test_marginals_brute_force_and_order():
+    regs = [2, 3, 2]
+    st = RNG.normal(size=2 ** 7) + 1j * RNG.normal(size=2 ** 7)
+    st /= np.linalg.norm(st)
+    s = CvdvOracle(regs)
+    s.initFromArray(st)
+    P = np.abs(st.reshape(4, 8, 4)) ** 2
+    np.testing.assert_allclose(s.m(1), P.sum(axis=(0, 2)), atol=1e-15)
+    np.testing.assert_allclose(s.m(2, 0), P.sum(axis=1).T, atol=1e-15)
+    np.testing.assert_allclose(s.m(1, 2, 0), np.transpose(P, (1, 2, 0)), atol=1e-15)
+    assert abs(s.m() - 1) < 1e-14
+
+
+def test_position_probs_ground_truth():
+    a = 0.6 + 0.8j
+    s, _ = _sim([8], [["setCoherent", 0, a]])
+    x = s.getXGrid(0)
+    g = np.exp(-(x - sqrt(2) * a.real) ** 2)
+    np.testing.assert_allclose(s.m(0), g / g.sum(), atol=3e-3)
+
+
+def test_conventions_differ_only_where_documented():
+    # cuda convention: no renormalisation at init, m() returns the squared norm
+    s, _ = _sim([4], [["setCoeffs", 0, np.full(16, 0.5)]], conv="cuda")
+    assert abs(s.m() - 4.0) < 1e-14
+    t, _ = _sim([4], [["setCoeffs", 0, np.full(16, 0.5)]], conv="torch")
+    assert abs(t.m() - 1.0) < 1e-14

@@ -1,0 +1,195 @@
+// cvdv_internal.h - shared host/device definitions of libcvdv_hip.
+//
+// State layout.  The state is one flat vector of 2^T amplitudes (T = sum of
+// register qubit counts), register 0 most significant, exactly as in the
+// reference (src/cuda/context.cu:27-38, src/cuda/kernels/utils.cuh:34-37).
+// Amplitudes are complex128 (double2) or, in the mixed mode, complex64 (float2)
+// with every phase angle, twiddle and probability sum kept in FP64.
+//
+// Lazy layout.  Each logical register r lives in a physical bit field
+// [pshift[r], pshift[r]+nq[r]) of the flat index and may be stored reflected
+// (physical j = N-1-logical j).  SWAP of equal registers and PARITY only update
+// this map; every kernel receives physical shifts.  Readouts that expose the
+// flat order (getState, state-state fidelity) first materialise the canonical
+// layout.
+//
+// Deferred diagonal phases.  Diagonal gates e^{i f(x)} are not launched on
+// their own: they accumulate in a DiagProg (phase terms in revolutions over
+// physical indices) that the next state-touching kernel applies to each
+// element as it is loaded.  A register's grid point x_j = (j-(N-1)/2) dx is
+// written x = m dx / 2 with the exact odd integer m = 2j-N+1, so every term is
+// a coefficient times an exact integer monomial (m, m^2, m^3, m1*m2).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#define CVDV_MAX_REG 32
+#define CVDV_MAX_TERMS 12
+
+enum CvdvPrecision { CVDV_C128 = 0, CVDV_MIXED = 1 };
+
+enum CvdvErr {
+    CVDV_OK = 0,
+    CVDV_ERR_ARG = 1,      // invalid argument (the reference prints and no-ops)
+    CVDV_ERR_STATE = 2,    // state not initialised
+    CVDV_ERR_HIP = 3,      // HIP runtime failure
+    CVDV_ERR_MEMORY = 4,   // allocation failure
+    CVDV_ERR_UNSUPPORTED = 5,
+};
+
+// ---------------------------------------------------------------- diag program
+enum DiagKind { DT_POLY = 1, DT_BILIN = 2, DT_ZBIT = 3 };
+
+struct DiagTerm {
+    int kind;
+    int sh1, nb1;  // field 1: m1 = 2*((g>>sh1)&(2^nb1-1)) + 1 - 2^nb1 ; ZBIT: bit sh1
+    int sh2, nb2;  // field 2 (BILIN)
+    int ctrl;      // control bit position (sign -1 when set), -1 = unconditional
+    double c1, c2, c3;  // POLY: c1 m + c2 m^2 + c3 m^3 ; BILIN: c1 m1 m2 ; ZBIT: +-c1
+};
+
+struct DiagProg {
+    int n;          // number of terms
+    int pad;
+    double grev;    // global phase, revolutions
+    DiagTerm t[CVDV_MAX_TERMS];
+};
+
+__host__ __device__ inline double cvdv_frac(double r) { return r - rint(r); }
+
+__device__ __forceinline__ double diag_field_m(uint64_t g, int sh, int nb) {
+    const uint64_t j = (g >> sh) & ((1ull << nb) - 1ull);
+    return (double)((int64_t)(2 * j) + 1 - ((int64_t)1 << nb));
+}
+
+// Phase of element g in revolutions, reduced to [-1/2, 1/2] term by term.
+__device__ __forceinline__ double diag_rev(const DiagProg& p, uint64_t g) {
+    double r = p.grev;
+    for (int i = 0; i < p.n; ++i) {
+        const DiagTerm& t = p.t[i];
+        double v;
+        if (t.kind == DT_POLY) {
+            const double m = diag_field_m(g, t.sh1, t.nb1);
+            v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
+        } else if (t.kind == DT_BILIN) {
+            const double m1 = diag_field_m(g, t.sh1, t.nb1);
+            const double m2 = diag_field_m(g, t.sh2, t.nb2);
+            v = t.c1 * (m1 * m2);  // m1*m2 is an exact integer below 2^53
+        } else {
+            v = ((g >> t.sh1) & 1ull) ? t.c1 : -t.c1;
+        }
+        if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) v = -v;
+        r += cvdv_frac(v);
+    }
+    return cvdv_frac(r);
+}
+
+// ---------------------------------------------------------------- complex helpers
+template <typename R> struct CT;
+template <> struct CT<float> { using type = float2; };
+template <> struct CT<double> { using type = double2; };
+
+template <typename C> __host__ __device__ inline C cmk(double x, double y) {
+    C r; r.x = (decltype(r.x))x; r.y = (decltype(r.y))y; return r;
+}
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+    return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+    return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+
+// e^{2 pi i f} for |f| <= 1/2.  Complex128: FP64 sincospi.  Mixed: the angle
+// is reduced in FP64 (diag_rev) and only the final sin/cos runs in FP32.
+__device__ __forceinline__ double2 unit_rev(double f, double) {
+    double s, c;
+    sincospi(2.0 * f, &s, &c);
+    return make_double2(c, s);
+}
+__device__ __forceinline__ float2 unit_rev(double f, float) {
+    float s, c;
+    sincospif((float)(2.0 * f), &s, &c);
+    return make_float2(c, s);
+}
+
+template <typename C>
+__device__ __forceinline__ C apply_diag(const DiagProg& p, uint64_t g, C v) {
+    using R = decltype(v.x);
+    return cmul(v, unit_rev(diag_rev(p, g), R(0)));
+}
+
+// ---------------------------------------------------------------- context
+struct TwiddleCache;  // fft.hip
+
+struct CVDVContext {
+    int numReg = 0;
+    int totalQbt = 0;
+    int prec = CVDV_C128;
+    size_t totalSize = 0;
+    int nq[CVDV_MAX_REG] = {0};
+    int flw[CVDV_MAX_REG] = {0};      // canonical shift (reference flwQbts)
+    double dx[CVDV_MAX_REG] = {0};
+    int pshift[CVDV_MAX_REG] = {0};   // physical shift of each logical register
+    int refl[CVDV_MAX_REG] = {0};     // 1 = stored reflected
+    void* buf[2] = {nullptr, nullptr};
+    int cur = 0;
+    bool initialized = false;
+    hipStream_t stream = nullptr;
+    DiagProg pending{};
+    TwiddleCache* tw = nullptr;
+    void* scratch = nullptr;          // readout scratch (grown on demand)
+    size_t scratchBytes = 0;
+
+    size_t elemBytes() const { return prec == CVDV_C128 ? 16 : 8; }
+    void* state() const { return buf[cur]; }
+    void* other();                    // allocates the ping-pong buffer on first use
+    bool canonical() const;
+};
+
+// ---------------------------------------------------------------- errors
+struct CvdvError {
+    int code;
+    std::string msg;
+};
+
+void cvdv_set_error(int code, const std::string& msg, bool print = true);
+void cvdv_check_hip(hipError_t e, const char* what);  // throws CvdvError
+
+#define HIPCHK(x) cvdv_check_hip((x), #x)
+
+// ---------------------------------------------------------------- internal API
+// gates.hip
+void ctx_push_term(CVDVContext* ctx, const DiagTerm& t);
+void ctx_flush(CVDVContext* ctx);              // apply pending diagonal in place
+void ctx_materialize(CVDVContext* ctx);        // flush + canonical layout
+void ctx_ft(CVDVContext* ctx, int reg, bool inverse);
+DiagProg ctx_take_pending(CVDVContext* ctx);
+void launch_diag(CVDVContext* ctx, const DiagProg& p);
+
+// fft.hip
+void fft_register(CVDVContext* ctx, int pshift, int nbits, bool inverse, const DiagProg& pre,
+                  double outScale);
+void fft_generic_c128(CVDVContext* ctx, double2* data, double2* tmp, uint64_t total, int shift,
+                      int nbits, bool inverse, double outScale);
+void twiddle_cache_free(TwiddleCache* tc);
+
+// measure.hip
+void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut);
+
+// scratch
+void* ctx_scratch(CVDVContext* ctx, size_t bytes);
+
+inline uint64_t ctx_ctrl_bit(const CVDVContext* c, int reg, int qubit) {
+    return (uint64_t)(c->pshift[reg] + c->nq[reg] - 1 - qubit);
+}

@@ -1,0 +1,463 @@
+// fft.hip - register-axis FFT for gfx950 (replaces the cuFFT plans and the
+// outer-block exec loop of src/cuda/context.cu:48-65 and gates.cu:37-52).
+//
+// A register field [sh, sh+n) of the flat index splits every element index as
+//     g = hi * (N*S) + j * S + lo,   S = 2^sh, N = 2^n,
+// and the FFT runs over j for every (hi, lo).  A workgroup stages a tile of
+// C consecutive lo columns x M rows x Hb consecutive hi blocks in LDS, runs a
+// Stockham FFT there (radix 8/4/2 butterflies in registers, per-stage twiddle
+// tables generated in FP64), and writes the tile back:
+//   * one HBM pass when a tile of >= 128 contiguous bytes per row fits in LDS;
+//   * otherwise the four-step split N = N1 * N2 (j = j1 + N1 j2):
+//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2};
+//       pass 2 (out of place, transposing): FFT over j1, written at k2 + N2 k1.
+// Pending diagonal phases are applied as elements are loaded by the first
+// pass and the 1/sqrt(N) normalisation as they are stored by the last.
+
+#include <cmath>
+#include <map>
+#include <vector>
+
+#include "cvdv_internal.h"
+
+static const long double PI_L = 3.141592653589793238462643383279502884L;
+
+// ---------------------------------------------------------------- twiddles
+struct TwKey {
+    int logM;
+    int prec;
+    bool operator<(const TwKey& o) const { return logM != o.logM ? logM < o.logM : prec < o.prec; }
+};
+
+struct TwiddleCache {
+    std::map<TwKey, void*> stage;  // per-stage Stockham tables for length M
+    std::map<TwKey, void*> full;   // W_N^e, e in [0, N)
+};
+
+void twiddle_cache_free(TwiddleCache* tc) {
+    for (auto& kv : tc->stage) (void)hipFree(kv.second);
+    for (auto& kv : tc->full) (void)hipFree(kv.second);
+    delete tc;
+}
+
+// exp(-2 pi i e / N) from an 80-bit evaluation, rounded once to double.
+static void host_twiddle(uint64_t e, uint64_t N, double& c, double& s) {
+    const long double a = 2.0L * PI_L * (long double)e / (long double)N;
+    c = (double)cosl(a);
+    s = (double)-sinl(a);
+}
+
+struct StagePlan {
+    int nst = 0;
+    int rad[16];
+    int L[16];
+    int off[16];
+    int total = 0;  // table entries
+};
+
+static StagePlan stage_plan(int logM) {
+    StagePlan p;
+    int rem = logM, L = 1, off = 0;
+    while (rem > 0) {
+        const int r = rem >= 3 ? 3 : rem;  // radix 8, then a final 4 or 2
+        const int R = 1 << r;
+        p.rad[p.nst] = R;
+        p.L[p.nst] = L;
+        p.off[p.nst] = off;
+        off += L * (R - 1);
+        L *= R;
+        rem -= r;
+        p.nst++;
+    }
+    p.total = off;
+    return p;
+}
+
+static void* upload_table(const std::vector<double>& re, const std::vector<double>& im, int prec) {
+    const size_t n = re.size();
+    void* d = nullptr;
+    if (prec == CVDV_C128) {
+        std::vector<double2> h(n ? n : 1);
+        for (size_t i = 0; i < n; ++i) h[i] = make_double2(re[i], im[i]);
+        HIPCHK(hipMalloc(&d, h.size() * sizeof(double2)));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
+    } else {
+        std::vector<float2> h(n ? n : 1);
+        for (size_t i = 0; i < n; ++i) h[i] = make_float2((float)re[i], (float)im[i]);
+        HIPCHK(hipMalloc(&d, h.size() * sizeof(float2)));
+        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
+    }
+    return d;
+}
+
+static TwiddleCache* cache(CVDVContext* ctx) {
+    if (!ctx->tw) ctx->tw = new TwiddleCache();
+    return ctx->tw;
+}
+
+static void* stage_table(CVDVContext* ctx, int logM, int prec) {
+    TwiddleCache* tc = cache(ctx);
+    auto it = tc->stage.find({logM, prec});
+    if (it != tc->stage.end()) return it->second;
+    const StagePlan p = stage_plan(logM);
+    const uint64_t M = 1ull << logM;
+    std::vector<double> re(p.total), im(p.total);
+    for (int s = 0; s < p.nst; ++s) {
+        const int R = p.rad[s], L = p.L[s];
+        for (int j = 0; j < L; ++j)
+            for (int i = 1; i < R; ++i) {
+                // W_{R L}^{i j} = W_M^{i j M / (R L)}
+                const uint64_t e = (uint64_t)i * j * (M / ((uint64_t)R * L));
+                host_twiddle(e, M, re[p.off[s] + j * (R - 1) + i - 1], im[p.off[s] + j * (R - 1) + i - 1]);
+            }
+    }
+    void* d = upload_table(re, im, prec);
+    tc->stage[{logM, prec}] = d;
+    return d;
+}
+
+static void* full_table(CVDVContext* ctx, int logN, int prec) {
+    TwiddleCache* tc = cache(ctx);
+    auto it = tc->full.find({logN, prec});
+    if (it != tc->full.end()) return it->second;
+    const uint64_t N = 1ull << logN;
+    std::vector<double> re(N), im(N);
+    for (uint64_t e = 0; e < N; ++e) host_twiddle(e, N, re[e], im[e]);
+    void* d = upload_table(re, im, prec);
+    tc->full[{logN, prec}] = d;
+    return d;
+}
+
+// ---------------------------------------------------------------- pass descriptor
+struct FftPass {
+    int M, C, Hb, rowFastIn, rowFastOut;
+    int nst;
+    int rad[6], L[6], off[6];
+    int tw4, tw4Shift, tw4LogN;
+    int usePre;
+    uint64_t nLo, nH2, nH1;
+    uint64_t inH1, inH2, inR;
+    uint64_t outH1, outH2, outR;
+    double outScale;
+};
+
+// ---------------------------------------------------------------- butterflies
+template <bool INV, typename C>
+__device__ __forceinline__ C mul_w(C a, C w) {
+    return INV ? cmulc(a, w) : cmul(a, w);
+}
+
+template <bool INV, typename C>
+__device__ __forceinline__ C rot_mi(C a) {  // a * (-i) forward, a * (+i) inverse
+    C r;
+    if (INV) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
+    return r;
+}
+
+template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
+template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
+
+template <bool INV, typename C>
+__device__ __forceinline__ void dft2(C* x) {
+    const C a = x[0], b = x[1];
+    x[0] = cadd(a, b);
+    x[1] = csub(a, b);
+}
+
+template <bool INV, typename C>
+__device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3) {
+    const C t0 = cadd(x0, x2), t1 = csub(x0, x2);
+    const C t2 = cadd(x1, x3), t3 = rot_mi<INV>(csub(x1, x3));
+    x0 = cadd(t0, t2);
+    x2 = csub(t0, t2);
+    x1 = cadd(t1, t3);
+    x3 = csub(t1, t3);
+}
+
+template <bool INV, typename C>
+__device__ __forceinline__ void dft8(C* x) {
+    using R = decltype(x[0].x);
+    C a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6];
+    C b0 = x[1], b1 = x[3], b2 = x[5], b3 = x[7];
+    dft4<INV>(a0, a1, a2, a3);
+    dft4<INV>(b0, b1, b2, b3);
+    const R h = (R)0.70710678118654752440;
+    // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = (-1 - i)/sqrt2 (forward)
+    C w1b, w3b;
+    if (!INV) {
+        w1b.x = h * (b1.x + b1.y); w1b.y = h * (b1.y - b1.x);
+        w3b.x = h * (b3.y - b3.x); w3b.y = -h * (b3.x + b3.y);
+    } else {
+        w1b.x = h * (b1.x - b1.y); w1b.y = h * (b1.y + b1.x);
+        w3b.x = -h * (b3.x + b3.y); w3b.y = h * (b3.x - b3.y);
+    }
+    const C w2b = rot_mi<INV>(b2);
+    x[0] = cadd(a0, b0); x[4] = csub(a0, b0);
+    x[1] = cadd(a1, w1b); x[5] = csub(a1, w1b);
+    x[2] = cadd(a2, w2b); x[6] = csub(a2, w2b);
+    x[3] = cadd(a3, w3b); x[7] = csub(a3, w3b);
+}
+
+template <int RP, bool INV, typename C>
+__device__ __forceinline__ void dft_r(C* x) {
+    if (RP == 2) dft2<INV>(x);
+    else if (RP == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
+    else dft8<INV>(x);
+}
+
+// One Stockham stage: each thread owns 8/RP butterflies of radix RP.
+template <int RP, bool INV, typename C>
+__device__ __forceinline__ void stockham_stage(C* lds, const FftPass& d, int s, const C* __restrict__ tw,
+                                               int tid, int nt) {
+    constexpr int NB = 8 / RP;
+    const int M = d.M, Cc = d.C, L = d.L[s], Mq = M / RP;
+    C x[8];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int b = tid + k * nt;
+        const int c = b % Cc;
+        const int rest = b / Cc;
+        const int q = rest % Mq;
+        const int base = (rest / Mq) * M;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[(base + q + i * Mq) * Cc + c];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const int b = tid + k * nt;
+        const int c = b % Cc;
+        const int rest = b / Cc;
+        const int q = rest % Mq;
+        const int base = (rest / Mq) * M;
+        const int j = q % L;
+        if (L > 1) {
+            const C* t = tw + d.off[s] + j * (RP - 1);
+#pragma unroll
+            for (int i = 1; i < RP; ++i) x[k * RP + i] = mul_w<INV>(x[k * RP + i], t[i - 1]);
+        }
+        dft_r<RP, INV>(x + k * RP);
+        const int ob = base + (q / L) * RP * L + j;
+#pragma unroll
+        for (int m = 0; m < RP; ++m) lds[(ob + m * L) * Cc + c] = x[k * RP + m];
+    }
+    __syncthreads();
+}
+
+template <typename R, bool INV>
+__global__ void __launch_bounds__(1024)
+k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, DiagProg pre,
+           const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
+    using C = typename CT<R>::type;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const int M = d.M, Cc = d.C, Hb = d.Hb;
+    const int tile = M * Cc * Hb;
+
+    uint64_t b = blockIdx.x;
+    const uint64_t nLoBlk = d.nLo / Cc, nH2Blk = d.nH2 / Hb;
+    const uint64_t loBlk = b % nLoBlk;
+    b /= nLoBlk;
+    const uint64_t h2Blk = b % nH2Blk;
+    const uint64_t h1 = b / nH2Blk;
+    const uint64_t lo0 = loBlk * Cc;
+    const uint64_t inBase = h1 * d.inH1 + h2Blk * Hb * d.inH2 + lo0;
+    const uint64_t outBase = h1 * d.outH1 + h2Blk * Hb * d.outH2 + lo0;
+
+    for (int e = tid; e < tile; e += nt) {
+        const int c = e % Cc;
+        const int rest = e / Cc;
+        int row, hb;
+        if (d.rowFastIn) { row = rest % M; hb = rest / M; } else { hb = rest % Hb; row = rest / Hb; }
+        const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
+        C v = in[g];
+        if (d.usePre) v = apply_diag(pre, g, v);
+        lds[(hb * M + row) * Cc + c] = v;
+    }
+    __syncthreads();
+
+    for (int s = 0; s < d.nst; ++s) {
+        const int rp = d.rad[s];
+        if (rp == 8) stockham_stage<8, INV>(lds, d, s, tw, tid, nt);
+        else if (rp == 4) stockham_stage<4, INV>(lds, d, s, tw, tid, nt);
+        else stockham_stage<2, INV>(lds, d, s, tw, tid, nt);
+    }
+
+    const R scale = (R)d.outScale;
+    const uint64_t nmask = (1ull << d.tw4LogN) - 1ull;
+    for (int e = tid; e < tile; e += nt) {
+        const int c = e % Cc;
+        const int rest = e / Cc;
+        int row, hb;
+        if (d.rowFastOut) { row = rest % M; hb = rest / M; } else { hb = rest % Hb; row = rest / Hb; }
+        C v = lds[(hb * M + row) * Cc + c];
+        if (d.tw4) {
+            const uint64_t j1 = (lo0 + c) >> d.tw4Shift;
+            v = mul_w<INV>(v, tw4[(j1 * (uint64_t)row) & nmask]);
+        }
+        v.x *= scale;
+        v.y *= scale;
+        out[outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c] = v;
+    }
+}
+
+// Direct DFT for very small states (fewer than 8 elements per tile).
+template <typename R, bool INV>
+__global__ void k_dft_naive(const typename CT<R>::type* in, typename CT<R>::type* out, uint64_t total,
+                            int sh, int nb, DiagProg pre, int usePre, double scale,
+                            const typename CT<R>::type* __restrict__ twN) {
+    using C = typename CT<R>::type;
+    const uint64_t M = 1ull << nb, fmask = (M - 1) << sh;
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = (g >> sh) & (M - 1);
+        const uint64_t base = g & ~fmask;
+        double ax = 0, ay = 0;
+        for (uint64_t j = 0; j < M; ++j) {
+            const uint64_t gi = base | (j << sh);
+            C v = in[gi];
+            if (usePre) v = apply_diag(pre, gi, v);
+            const C w = twN[(j * k) & (M - 1)];
+            const C p = mul_w<INV>(v, w);
+            ax += (double)p.x;
+            ay += (double)p.y;
+        }
+        out[g] = cmk<C>(ax * scale, ay * scale);
+    }
+}
+
+// ---------------------------------------------------------------- planning
+struct PassGeom {
+    int M;
+    uint64_t nLo, nH2, nH1;
+    uint64_t inH1, inH2, inR, outH1, outH2, outR;
+};
+
+static int maxTile(int prec) { return prec == CVDV_C128 ? 4096 : 8192; }
+static int colWidth(int prec) { return prec == CVDV_C128 ? 8 : 16; }
+
+static bool fit_pass(const PassGeom& g, int prec, FftPass& d) {
+    const uint64_t W = colWidth(prec);
+    const uint64_t C = g.nLo < W ? g.nLo : W;
+    const uint64_t tileCM = C * (uint64_t)g.M;
+    if (tileCM > (uint64_t)maxTile(prec)) return false;
+    uint64_t Hb = 1;
+    const uint64_t target = 2048;
+    while (Hb * 2 <= g.nH2 && tileCM * Hb * 2 <= target) Hb *= 2;
+    if (tileCM * Hb < 8) return false;
+    d.M = g.M;
+    d.C = (int)C;
+    d.Hb = (int)Hb;
+    d.nLo = g.nLo;
+    d.nH2 = g.nH2;
+    d.nH1 = g.nH1;
+    d.inH1 = g.inH1; d.inH2 = g.inH2; d.inR = g.inR;
+    d.outH1 = g.outH1; d.outH2 = g.outH2; d.outR = g.outR;
+    d.rowFastIn = g.inR <= g.inH2 ? 1 : 0;
+    d.rowFastOut = g.outR <= g.outH2 ? 1 : 0;
+    return true;
+}
+
+template <typename R>
+static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
+                        FftPass d, const DiagProg& pre, bool inverse, int prec) {
+    using C = typename CT<R>::type;
+    const StagePlan sp = stage_plan(__builtin_ctz((unsigned)d.M));
+    d.nst = sp.nst;
+    for (int s = 0; s < sp.nst; ++s) {
+        d.rad[s] = sp.rad[s];
+        d.L[s] = sp.L[s];
+        d.off[s] = sp.off[s];
+    }
+    const C* tw = (const C*)stage_table(ctx, __builtin_ctz((unsigned)d.M), prec);
+    const C* tw4 = d.tw4 ? (const C*)full_table(ctx, d.tw4LogN, prec) : tw;
+    const int tile = d.M * d.C * d.Hb;
+    const int nt = tile / 8;
+    const uint64_t blocks = d.nH1 * (d.nH2 / d.Hb) * (d.nLo / d.C);
+    const size_t lds = (size_t)tile * sizeof(C);
+    if (inverse)
+        hipLaunchKernelGGL((k_fft_tile<R, true>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
+                           in, out, d, pre, tw, tw4);
+    else
+        hipLaunchKernelGGL((k_fft_tile<R, false>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
+                           in, out, d, pre, tw, tw4);
+    HIPCHK(hipGetLastError());
+}
+
+// Runs the FFT over field [sh, sh+nb) of a buffer of `total` elements.
+// Returns true when the result landed in `tmp` (caller swaps buffers).
+template <typename R, typename TmpFn>
+static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, uint64_t total,
+                    int sh, int nb, bool inverse, const DiagProg& pre, double outScale, int prec) {
+    using C = typename CT<R>::type;
+    const uint64_t N = 1ull << nb, S = 1ull << sh;
+    const uint64_t H = total / (N * S);
+    const int usePre = (pre.n > 0 || pre.grev != 0.0) ? 1 : 0;
+
+    // single pass: rows j (stride S), lo in [0,S), hi blocks in h2 (stride N*S)
+    PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
+    g1.inH1 = 0; g1.outH1 = 0;
+    FftPass d{};
+    if (N <= (uint64_t)maxTile(prec) && fit_pass(g1, prec, d)) {
+        d.tw4 = 0; d.tw4Shift = 0; d.tw4LogN = 0;
+        d.usePre = usePre;
+        d.outScale = outScale;
+        launch_pass<R>(ctx, data, data, d, pre, inverse, prec);
+        return false;
+    }
+    if (total < 4096 || N <= 64) {
+        // tiny state: direct DFT, out of place
+        const C* twN = (const C*)full_table(ctx, nb, prec);
+        C* tmp = tmpFn();
+        uint64_t blocks = (total + 255) / 256;
+        if (inverse)
+            hipLaunchKernelGGL((k_dft_naive<R, true>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                               data, tmp, total, sh, nb, pre, usePre, outScale, twN);
+        else
+            hipLaunchKernelGGL((k_dft_naive<R, false>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
+                               data, tmp, total, sh, nb, pre, usePre, outScale, twN);
+        HIPCHK(hipGetLastError());
+        return true;
+    }
+    // four-step: j = j1 + N1 j2
+    const int n1 = nb / 2, n2 = nb - n1;
+    const uint64_t N1 = 1ull << n1, N2 = 1ull << n2;
+    // pass 1: FFT over j2 (stride N1*S), columns (j1, lo) contiguous, in place
+    PassGeom p1{(int)N2, N1 * S, 1, H, N * S, 0, N1 * S, N * S, 0, N1 * S};
+    FftPass d1{};
+    // pass 2: FFT over j1 (stride S), h2 = k2 (in stride N1*S, out stride S), out rows stride N2*S
+    PassGeom p2{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, S, N2 * S};
+    FftPass d2{};
+    if (!fit_pass(p1, prec, d1) || !fit_pass(p2, prec, d2))
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
+    d1.tw4 = 1; d1.tw4Shift = sh; d1.tw4LogN = nb;
+    d1.usePre = usePre;
+    d1.outScale = 1.0;
+    d2.tw4 = 0; d2.tw4Shift = 0; d2.tw4LogN = 0;
+    d2.usePre = 0;
+    d2.outScale = outScale;
+    C* tmp = tmpFn();
+    launch_pass<R>(ctx, data, data, d1, pre, inverse, prec);
+    launch_pass<R>(ctx, data, tmp, d2, DiagProg{}, inverse, prec);
+    return true;
+}
+
+void fft_register(CVDVContext* ctx, int pshift, int nbits, bool inverse, const DiagProg& pre,
+                  double outScale) {
+    bool swapped;
+    if (ctx->prec == CVDV_C128)
+        swapped = fft_run<double>(ctx, (double2*)ctx->state(), [ctx] { return (double2*)ctx->other(); },
+                                  ctx->totalSize, pshift, nbits, inverse, pre, outScale, CVDV_C128);
+    else
+        swapped = fft_run<float>(ctx, (float2*)ctx->state(), [ctx] { return (float2*)ctx->other(); },
+                                 ctx->totalSize, pshift, nbits, inverse, pre, outScale, CVDV_MIXED);
+    if (swapped) ctx->cur = 1 - ctx->cur;
+}
+
+void fft_generic_c128(CVDVContext* ctx, double2* data, double2* tmp, uint64_t total, int shift,
+                      int nbits, bool inverse, double outScale) {
+    if (fft_run<double>(ctx, data, [tmp] { return tmp; }, total, shift, nbits, inverse, DiagProg{},
+                        outScale, CVDV_C128))
+        HIPCHK(hipMemcpyAsync(data, tmp, total * sizeof(double2), hipMemcpyDeviceToDevice, ctx->stream));
+}

@@ -1,0 +1,651 @@
+// gates.hip - gate compiler and the element-wise / pair kernels.
+//
+// Every C entry point issues the same decomposition as the reference
+// (src/cuda/gates.cu:19-666, same turn-count rounding, same 1e-12 / 1e-15
+// guards), but diagonal factors are not launched one by one: they are added
+// to the context's pending DiagProg and applied by the next kernel that loads
+// the state (an FFT pass, a qubit-pair pass, a flush).  Parity and register
+// swap are recorded in the lazy layout map instead of moving data.
+
+#include <cmath>
+#include <cstdio>
+#include <string>
+
+#include "cvdv_internal.h"
+#include "../../include/cvdv_hip.h"
+
+static const double PI = 3.14159265358979323846;
+static const double SQRT2 = 1.41421356237309504880;
+
+// ---------------------------------------------------------------- kernels
+template <typename C>
+__global__ void k_diag(C* __restrict__ s, uint64_t total, DiagProg p) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x)
+        s[g] = apply_diag(p, g, s[g]);
+}
+
+// 2x2 unitary on the pair (bit = 0, bit = 1); pending phases applied on load.
+template <typename C>
+__global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, C m00, C m01, C m10, C m11,
+                        DiagProg p, int usep) {
+    const uint64_t lowmask = (1ull << bit) - 1ull;
+    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
+         pid += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);
+        const uint64_t i1 = i0 | (1ull << bit);
+        C a = s[i0], b = s[i1];
+        if (usep) {
+            a = apply_diag(p, i0, a);
+            b = apply_diag(p, i1, b);
+        }
+        C na = cmul(m00, a), nb = cmul(m10, a);
+        const C t0 = cmul(m01, b), t1 = cmul(m11, b);
+        na.x += t0.x; na.y += t0.y;
+        nb.x += t1.x; nb.y += t1.y;
+        s[i0] = na;
+        s[i1] = nb;
+    }
+}
+
+// Conditional reflection j <-> N-1-j of the field [sh, sh+nb) where the
+// control bit of the logically lower partner is 1 (reference semantics,
+// src/cuda/kernels/gates.cuh:134-157 and torchCvdv.py:122-137).
+template <typename C>
+__global__ void k_cparity(C* __restrict__ s, uint64_t npairs, int sh, int nb, int ctrlPos,
+                          int ctrlFlip, int tgtRefl, DiagProg p, int usep) {
+    const int top = sh + nb - 1;
+    const uint64_t lowmask = (1ull << top) - 1ull;
+    const uint64_t fmask = ((1ull << nb) - 1ull) << sh;
+    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
+         pid += (uint64_t)gridDim.x * blockDim.x) {
+        const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);  // top field bit 0
+        const uint64_t i1 = i0 ^ fmask;
+        C a = s[i0], b = s[i1];
+        if (usep) {
+            a = apply_diag(p, i0, a);
+            b = apply_diag(p, i1, b);
+        }
+        const uint64_t lower = tgtRefl ? i1 : i0;
+        const int cb = (int)((lower >> ctrlPos) & 1ull) ^ ctrlFlip;
+        if (cb) {
+            s[i0] = b;
+            s[i1] = a;
+        } else if (usep) {
+            s[i0] = a;
+            s[i1] = b;
+        }
+    }
+}
+
+static unsigned grid_for(uint64_t total, unsigned block = 256) {
+    uint64_t b = (total + block - 1) / block;
+    if (b > 65536ull * 8) b = 65536ull * 8;
+    return (unsigned)(b ? b : 1);
+}
+
+static inline bool prog_active(const DiagProg& p) { return p.n > 0 || p.grev != 0.0; }
+
+// ---------------------------------------------------------------- pending program
+DiagProg ctx_take_pending(CVDVContext* ctx) {
+    DiagProg p = ctx->pending;
+    ctx->pending = DiagProg{};
+    return p;
+}
+
+void launch_diag(CVDVContext* ctx, const DiagProg& p) {
+    const uint64_t n = ctx->totalSize;
+    if (ctx->prec == CVDV_C128)
+        hipLaunchKernelGGL(k_diag<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (double2*)ctx->state(), n, p);
+    else
+        hipLaunchKernelGGL(k_diag<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (float2*)ctx->state(), n, p);
+    HIPCHK(hipGetLastError());
+}
+
+void ctx_flush(CVDVContext* ctx) {
+    if (!prog_active(ctx->pending)) return;
+    DiagProg p = ctx_take_pending(ctx);
+    launch_diag(ctx, p);
+}
+
+static bool same_slot(const DiagTerm& a, const DiagTerm& b) {
+    return a.kind == b.kind && a.sh1 == b.sh1 && a.nb1 == b.nb1 && a.sh2 == b.sh2 &&
+           a.nb2 == b.nb2 && a.ctrl == b.ctrl;
+}
+
+static void prog_add(DiagProg& p, const DiagTerm& t) {
+    for (int i = 0; i < p.n; ++i) {
+        if (same_slot(p.t[i], t)) {
+            p.t[i].c1 += t.c1;
+            p.t[i].c2 += t.c2;
+            p.t[i].c3 += t.c3;
+            return;
+        }
+    }
+    p.t[p.n++] = t;
+}
+
+void ctx_push_term(CVDVContext* ctx, const DiagTerm& t0) {
+    DiagTerm t = t0;
+    if (t.kind == DT_BILIN && t.sh2 < t.sh1) {
+        std::swap(t.sh1, t.sh2);
+        std::swap(t.nb1, t.nb2);
+    }
+    bool found = false;
+    for (int i = 0; i < ctx->pending.n; ++i) found |= same_slot(ctx->pending.t[i], t);
+    if (!found && ctx->pending.n == CVDV_MAX_TERMS) ctx_flush(ctx);
+    prog_add(ctx->pending, t);
+}
+
+// A logical diagonal term over register r (radians per x^k) -> physical term.
+static DiagTerm poly_term(CVDVContext* ctx, int r, double c1, double c2, double c3, int ctrlReg,
+                          int ctrlQubit) {
+    DiagTerm t{};
+    t.kind = DT_POLY;
+    t.sh1 = ctx->pshift[r];
+    t.nb1 = ctx->nq[r];
+    const double N = (double)((uint64_t)1 << ctx->nq[r]);
+    const double dx = ctx->dx[r];
+    // x = m dx / 2; revolutions = angle / (2 pi); dx^2 = 2 pi / N exactly.
+    t.c1 = c1 * dx / (4.0 * PI);
+    t.c2 = c2 / (4.0 * N);
+    t.c3 = c3 * dx * dx * dx / (16.0 * PI);
+    if (ctx->refl[r]) {  // logical m = -physical m
+        t.c1 = -t.c1;
+        t.c3 = -t.c3;
+    }
+    t.ctrl = -1;
+    if (ctrlReg >= 0) {
+        t.ctrl = (int)ctx_ctrl_bit(ctx, ctrlReg, ctrlQubit);
+        if (ctx->refl[ctrlReg]) {  // logical control bit = NOT physical bit
+            t.c1 = -t.c1;
+            t.c2 = -t.c2;
+            t.c3 = -t.c3;
+        }
+    }
+    return t;
+}
+
+static DiagTerm bilin_term(CVDVContext* ctx, int r1, int r2, double coeff, int ctrlReg,
+                           int ctrlQubit) {
+    DiagTerm t{};
+    t.kind = DT_BILIN;
+    t.sh1 = ctx->pshift[r1];
+    t.nb1 = ctx->nq[r1];
+    t.sh2 = ctx->pshift[r2];
+    t.nb2 = ctx->nq[r2];
+    if (ctx->nq[r1] == ctx->nq[r2])
+        t.c1 = coeff / (4.0 * (double)((uint64_t)1 << ctx->nq[r1]));
+    else
+        t.c1 = coeff * ctx->dx[r1] * ctx->dx[r2] / (8.0 * PI);
+    if (ctx->refl[r1] ^ ctx->refl[r2]) t.c1 = -t.c1;
+    t.ctrl = -1;
+    if (ctrlReg >= 0) {
+        t.ctrl = (int)ctx_ctrl_bit(ctx, ctrlReg, ctrlQubit);
+        if (ctx->refl[ctrlReg]) t.c1 = -t.c1;
+    }
+    return t;
+}
+
+// ---------------------------------------------------------------- primitive ops
+void ctx_ft(CVDVContext* ctx, int reg, bool inverse) {
+    const int nb = ctx->nq[reg];
+    if (nb == 0) return;  // N = 1: the transform is the identity
+    const uint64_t N = (uint64_t)1 << nb;
+    const double sgn = inverse ? -1.0 : 1.0;
+    // Centred DFT: F[k,j] = N^-1/2 exp(-2 pi i m_j m_k / (4N)) =
+    //   e^{2 pi i (N-1)^2/(4N)} D FFT D / sqrt(N),  D = diag e^{2 pi i (N-1) m/(4N)}
+    // (gates.cu:19-67: pre phase, FFT, post phase, 1/sqrt(N), global phase).
+    DiagTerm tw{};
+    tw.kind = DT_POLY;
+    tw.sh1 = ctx->pshift[reg];
+    tw.nb1 = nb;
+    tw.ctrl = -1;
+    tw.c1 = sgn * (double)(N - 1) / (4.0 * (double)N);
+    DiagProg pre = ctx_take_pending(ctx);
+    if (pre.n == CVDV_MAX_TERMS) {
+        bool found = false;
+        for (int i = 0; i < pre.n; ++i) found |= same_slot(pre.t[i], tw);
+        if (!found) {
+            launch_diag(ctx, pre);
+            pre = DiagProg{};
+        }
+    }
+    prog_add(pre, tw);
+    fft_register(ctx, ctx->pshift[reg], nb, inverse, pre, 1.0 / std::sqrt((double)N));
+    ctx->pending = DiagProg{};
+    prog_add(ctx->pending, tw);
+    const double gphase = (double)(N - 1) * (double)(N - 1) / (4.0 * (double)N);
+    ctx->pending.grev = cvdv_frac(sgn * gphase);
+}
+
+static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) {
+    // m = {m00.re, m00.im, m01.re, m01.im, m10..., m11...}
+    double mm[8];
+    for (int i = 0; i < 8; ++i) mm[i] = m[i];
+    if (ctx->refl[reg]) {  // logical bit = NOT physical bit: X M X
+        const double t[8] = {m[6], m[7], m[4], m[5], m[2], m[3], m[0], m[1]};
+        for (int i = 0; i < 8; ++i) mm[i] = t[i];
+    }
+    const int bit = (int)ctx_ctrl_bit(ctx, reg, qubit);
+    DiagProg p = ctx_take_pending(ctx);
+    const int usep = prog_active(p) ? 1 : 0;
+    const uint64_t np = ctx->totalSize / 2;
+    if (ctx->prec == CVDV_C128)
+        hipLaunchKernelGGL(k_qubit<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+                           (double2*)ctx->state(), np, bit, make_double2(mm[0], mm[1]),
+                           make_double2(mm[2], mm[3]), make_double2(mm[4], mm[5]),
+                           make_double2(mm[6], mm[7]), p, usep);
+    else
+        hipLaunchKernelGGL(k_qubit<float2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+                           (float2*)ctx->state(), np, bit, make_float2(mm[0], mm[1]),
+                           make_float2(mm[2], mm[3]), make_float2(mm[4], mm[5]),
+                           make_float2(mm[6], mm[7]), p, usep);
+    HIPCHK(hipGetLastError());
+}
+
+static void zbit(CVDVContext* ctx, int reg, int qubit, double theta) {
+    // Rz(theta) = diag(e^{-i theta/2}, e^{i theta/2}) (gates.cuh:88-91)
+    DiagTerm t{};
+    t.kind = DT_ZBIT;
+    t.sh1 = (int)ctx_ctrl_bit(ctx, reg, qubit);
+    t.nb1 = 1;
+    t.ctrl = -1;
+    t.c1 = theta / (4.0 * PI);
+    if (ctx->refl[reg]) t.c1 = -t.c1;
+    ctx_push_term(ctx, t);
+}
+
+static void pauli(CVDVContext* ctx, int reg, int qubit, int axis, double theta) {
+    if (axis != 0 && axis != 1) {  // gates.cuh:88: any other axis value is Z
+        zbit(ctx, reg, qubit, theta);
+        return;
+    }
+    const double c = std::cos(theta / 2.0), s = std::sin(theta / 2.0);
+    if (axis == 0) {  // Rx = [[c, -is], [-is, c]]
+        const double m[8] = {c, 0, 0, -s, 0, -s, c, 0};
+        qubit_gate(ctx, reg, qubit, m);
+    } else {  // Ry = [[c, -s], [s, c]]
+        const double m[8] = {c, 0, -s, 0, s, 0, c, 0};
+        qubit_gate(ctx, reg, qubit, m);
+    }
+}
+
+static void hadamard(CVDVContext* ctx, int reg, int qubit) {
+    const double h = 1.0 / SQRT2;
+    const double m[8] = {h, 0, h, 0, h, 0, -h, 0};
+    qubit_gate(ctx, reg, qubit, m);
+}
+
+static void parity(CVDVContext* ctx, int reg) { ctx->refl[reg] ^= 1; }
+
+static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
+    const int nb = ctx->nq[tgt];
+    if (nb == 0) return;
+    DiagProg p = ctx_take_pending(ctx);
+    const int usep = prog_active(p) ? 1 : 0;
+    const uint64_t np = ctx->totalSize / 2;
+    const int cpos = (int)ctx_ctrl_bit(ctx, creg, cq);
+    if (ctx->prec == CVDV_C128)
+        hipLaunchKernelGGL(k_cparity<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+                           (double2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
+                           ctx->refl[tgt], p, usep);
+    else
+        hipLaunchKernelGGL(k_cparity<float2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+                           (float2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
+                           ctx->refl[tgt], p, usep);
+    HIPCHK(hipGetLastError());
+}
+
+static void swap_regs(CVDVContext* ctx, int r1, int r2) {
+    std::swap(ctx->pshift[r1], ctx->pshift[r2]);
+    std::swap(ctx->refl[r1], ctx->refl[r2]);
+}
+
+static void phase_poly(CVDVContext* ctx, int r, double c1, double c2, double c3, int creg = -1,
+                       int cq = 0) {
+    ctx_push_term(ctx, poly_term(ctx, r, c1, c2, c3, creg, cq));
+}
+
+static void phase_bilin(CVDVContext* ctx, int r1, int r2, double c, int creg = -1, int cq = 0) {
+    ctx_push_term(ctx, bilin_term(ctx, r1, r2, c, creg, cq));
+}
+
+// ---------------------------------------------------------------- validation
+#define CVDV_TRY try {
+#define CVDV_CATCH()                                                         \
+    }                                                                        \
+    catch (const CvdvError& e) {                                             \
+        cvdv_set_error(e.code, e.msg);                                       \
+    }                                                                        \
+    catch (...) {                                                            \
+        cvdv_set_error(CVDV_ERR_HIP, "unexpected internal error");           \
+    }
+
+static bool ready(CVDVContext* ctx) {
+    if (!ctx) return false;
+    if (!ctx->initialized) {
+        cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
+        return false;
+    }
+    return true;
+}
+
+static bool reg_ok(CVDVContext* ctx, int r, const char* what = "Invalid register index") {
+    if (r < 0 || r >= ctx->numReg) {
+        cvdv_set_error(CVDV_ERR_ARG, std::string(what) + ": " + std::to_string(r));
+        return false;
+    }
+    return true;
+}
+
+static bool ctrl_ok(CVDVContext* ctx, int creg, int cq) {
+    if (!reg_ok(ctx, creg, "Invalid control register index")) return false;
+    if (cq < 0 || cq >= ctx->nq[creg]) {
+        cvdv_set_error(CVDV_ERR_ARG, "Invalid control qubit index: " + std::to_string(cq));
+        return false;
+    }
+    return true;
+}
+
+static bool qubit_ok(CVDVContext* ctx, int r, int q) {
+    if (!reg_ok(ctx, r)) return false;
+    if (q < 0 || q >= ctx->nq[r]) {
+        cvdv_set_error(CVDV_ERR_ARG, "Invalid qubit index: " + std::to_string(q));
+        return false;
+    }
+    return true;
+}
+
+static bool two_regs_ok(CVDVContext* ctx, int r1, int r2) {
+    if (r1 < 0 || r1 >= ctx->numReg || r2 < 0 || r2 >= ctx->numReg) {
+        cvdv_set_error(CVDV_ERR_ARG, "Invalid register indices: " + std::to_string(r1) + ", " +
+                                         std::to_string(r2));
+        return false;
+    }
+    return true;
+}
+
+// ---------------------------------------------------------------- composite helpers
+static void rotation_small(CVDVContext* ctx, int r, double th) {  // gates.cu:572-584
+    if (std::fabs(th) < 1e-15) return;
+    const double tq = std::tan(th / 2.0), st = std::sin(th);
+    phase_poly(ctx, r, 0, -0.5 * tq, 0);
+    ctx_ft(ctx, r, false);
+    phase_poly(ctx, r, 0, -0.5 * st, 0);
+    ctx_ft(ctx, r, true);
+    phase_poly(ctx, r, 0, -0.5 * tq, 0);
+}
+
+static void crotation_small(CVDVContext* ctx, int t, int c, int q, double th) {  // gates.cu:598-611
+    if (std::fabs(th) < 1e-15) return;
+    const double tq = std::tan(th / 2.0), st = std::sin(th);
+    phase_poly(ctx, t, 0, -0.5 * tq, 0, c, q);
+    ctx_ft(ctx, t, false);
+    phase_poly(ctx, t, 0, -0.5 * st, 0, c, q);
+    ctx_ft(ctx, t, true);
+    phase_poly(ctx, t, 0, -0.5 * tq, 0, c, q);
+}
+
+static void bs_small(CVDVContext* ctx, int r1, int r2, double th, int c = -1, int q = 0) {
+    // gates.cu:614-636 (and 640-666 with a control)
+    if (std::fabs(th) < 1e-15) return;
+    const double cq = -std::tan(0.25 * th), cp = -std::sin(0.5 * th);
+    phase_bilin(ctx, r1, r2, cq, c, q);
+    ctx_ft(ctx, r1, false);
+    ctx_ft(ctx, r2, false);
+    phase_bilin(ctx, r1, r2, cp, c, q);
+    ctx_ft(ctx, r1, true);
+    ctx_ft(ctx, r2, true);
+    phase_bilin(ctx, r1, r2, cq, c, q);
+}
+
+// ---------------------------------------------------------------- C ABI
+extern "C" {
+
+void cvdvFtQ2P(CVDVContext* ctx, int regIdx) {
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY ctx_ft(ctx, regIdx, false); CVDV_CATCH()
+}
+
+void cvdvFtP2Q(CVDVContext* ctx, int regIdx) {
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY ctx_ft(ctx, regIdx, true); CVDV_CATCH()
+}
+
+void cvdvDisplacement(CVDVContext* ctx, int regIdx, double betaRe, double betaIm) {
+    // gates.cu:124-161; the global phase e^{-i Re Im} is dropped there too
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY
+    if (std::fabs(betaIm) > 1e-12) phase_poly(ctx, regIdx, SQRT2 * betaIm, 0, 0);
+    if (std::fabs(betaRe) > 1e-12) {
+        ctx_ft(ctx, regIdx, false);
+        phase_poly(ctx, regIdx, -SQRT2 * betaRe, 0, 0);
+        ctx_ft(ctx, regIdx, true);
+    }
+    CVDV_CATCH()
+}
+
+void cvdvConditionalDisplacement(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit,
+                                 double alphaRe, double alphaIm) {
+    // gates.cu:163-204
+    if (!ready(ctx) || !reg_ok(ctx, targetReg, "Invalid target register index") ||
+        !ctrl_ok(ctx, ctrlReg, ctrlQubit))
+        return;
+    CVDV_TRY
+    if (std::fabs(alphaIm) > 1e-12) phase_poly(ctx, targetReg, SQRT2 * alphaIm, 0, 0, ctrlReg, ctrlQubit);
+    if (std::fabs(alphaRe) > 1e-12) {
+        ctx_ft(ctx, targetReg, false);
+        phase_poly(ctx, targetReg, -SQRT2 * alphaRe, 0, 0, ctrlReg, ctrlQubit);
+        ctx_ft(ctx, targetReg, true);
+    }
+    CVDV_CATCH()
+}
+
+void cvdvPauliRotation(CVDVContext* ctx, int regIdx, int qubitIdx, int axis, double theta) {
+    // gates.cu:206-219
+    if (!ready(ctx) || !qubit_ok(ctx, regIdx, qubitIdx)) return;
+    CVDV_TRY pauli(ctx, regIdx, qubitIdx, axis, theta); CVDV_CATCH()
+}
+
+void cvdvHadamard(CVDVContext* ctx, int regIdx, int qubitIdx) {
+    // gates.cu:221-233
+    if (!ready(ctx) || !qubit_ok(ctx, regIdx, qubitIdx)) return;
+    CVDV_TRY hadamard(ctx, regIdx, qubitIdx); CVDV_CATCH()
+}
+
+void cvdvParity(CVDVContext* ctx, int regIdx) {
+    // gates.cu:235-246: j -> N-1-j, recorded in the layout map
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    parity(ctx, regIdx);
+}
+
+void cvdvConditionalParity(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit) {
+    // gates.cu:248-268
+    if (!ready(ctx) || !reg_ok(ctx, targetReg, "Invalid target register index") ||
+        !ctrl_ok(ctx, ctrlReg, ctrlQubit))
+        return;
+    CVDV_TRY cparity(ctx, targetReg, ctrlReg, ctrlQubit); CVDV_CATCH()
+}
+
+void cvdvSwapRegisters(CVDVContext* ctx, int reg1, int reg2) {
+    // gates.cu:270-286: recorded in the layout map
+    if (!ready(ctx) || !two_regs_ok(ctx, reg1, reg2)) return;
+    if (ctx->nq[reg1] != ctx->nq[reg2]) {
+        cvdv_set_error(CVDV_ERR_ARG, "SWAP requires registers with same number of qubits");
+        return;
+    }
+    if (reg1 == reg2) return;
+    swap_regs(ctx, reg1, reg2);
+}
+
+void cvdvPhaseSquare(CVDVContext* ctx, int regIdx, double t) {
+    // gates.cu:288-300: e^{i t x^2}
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY phase_poly(ctx, regIdx, 0, t, 0); CVDV_CATCH()
+}
+
+void cvdvPhaseCubic(CVDVContext* ctx, int regIdx, double t) {
+    // gates.cu:302-314: e^{i t x^3}
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY phase_poly(ctx, regIdx, 0, 0, t); CVDV_CATCH()
+}
+
+void cvdvRotation(CVDVContext* ctx, int regIdx, double theta) {
+    // gates.cu:316-348: quarter turns by rint (half-even), then the 3-shear remainder
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY
+    const double theta0 = std::rint(theta / (PI / 2.0)) * (PI / 2.0);
+    const double rem = theta - theta0;
+    int qt = (int)std::rint(theta0 / (PI / 2.0));
+    qt = ((qt % 4) + 4) % 4;
+    if (qt == 1) ctx_ft(ctx, regIdx, false);
+    else if (qt == 2) parity(ctx, regIdx);
+    else if (qt == 3) ctx_ft(ctx, regIdx, true);
+    rotation_small(ctx, regIdx, rem);
+    CVDV_CATCH()
+}
+
+void cvdvConditionalRotation(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit,
+                             double theta) {
+    // gates.cu:350-392
+    if (!ready(ctx) || !reg_ok(ctx, targetReg, "Invalid target register index") ||
+        !ctrl_ok(ctx, ctrlReg, ctrlQubit))
+        return;
+    CVDV_TRY
+    const double theta0 = std::rint(theta / (PI / 2.0)) * (PI / 2.0);
+    const double rem = theta - theta0;
+    int qt = (int)std::rint(theta0 / (PI / 2.0));
+    qt = ((qt % 4) + 4) % 4;
+    if (qt == 1) {
+        ctx_ft(ctx, targetReg, false);
+        cparity(ctx, targetReg, ctrlReg, ctrlQubit);
+        zbit(ctx, ctrlReg, ctrlQubit, PI / 2);
+    } else if (qt == 2) {
+        parity(ctx, targetReg);
+        zbit(ctx, ctrlReg, ctrlQubit, PI);
+    } else if (qt == 3) {
+        ctx_ft(ctx, targetReg, true);
+        cparity(ctx, targetReg, ctrlReg, ctrlQubit);
+        zbit(ctx, ctrlReg, ctrlQubit, -PI / 2);
+    }
+    crotation_small(ctx, targetReg, ctrlReg, ctrlQubit, rem);
+    CVDV_CATCH()
+}
+
+void cvdvSqueeze(CVDVContext* ctx, int regIdx, double r) {
+    // gates.cu:394-421
+    if (!ready(ctx) || !reg_ok(ctx, regIdx)) return;
+    CVDV_TRY
+    const double eh = std::exp(0.5 * r), emh = std::exp(-0.5 * r);
+    const double sq = std::sqrt(std::fabs(std::exp(r) - 1.0));
+    const double sign = (r >= 0) ? 1.0 : -1.0;
+    ctx_ft(ctx, regIdx, false);
+    phase_poly(ctx, regIdx, 0, -0.5 * eh * sq, 0);
+    ctx_ft(ctx, regIdx, true);
+    phase_poly(ctx, regIdx, 0, 0.5 * emh * sq * sign, 0);
+    ctx_ft(ctx, regIdx, false);
+    phase_poly(ctx, regIdx, 0, 0.5 * emh * sq, 0);
+    ctx_ft(ctx, regIdx, true);
+    phase_poly(ctx, regIdx, 0, -0.5 * eh * sq * sign, 0);
+    CVDV_CATCH()
+}
+
+void cvdvConditionalSqueeze(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit, double r) {
+    // gates.cu:423-459 (r = 0 gives NaN, as in the reference)
+    if (!ready(ctx) || !reg_ok(ctx, targetReg, "Invalid target register index") ||
+        !ctrl_ok(ctx, ctrlReg, ctrlQubit))
+        return;
+    CVDV_TRY
+    const double ch = std::cosh(r), sh = std::sinh(r);
+    const double s = std::sqrt(2.0 * std::fabs(std::sinh(0.5 * r)));
+    phase_poly(ctx, targetReg, 0, 0.5 * s * ch, 0);
+    phase_poly(ctx, targetReg, 0, -0.5 * s * sh, 0, ctrlReg, ctrlQubit);
+    ctx_ft(ctx, targetReg, false);
+    phase_poly(ctx, targetReg, 0, 0.5 * (ch - 1) / s, 0);
+    phase_poly(ctx, targetReg, 0, 0.5 * sh / s, 0, ctrlReg, ctrlQubit);
+    ctx_ft(ctx, targetReg, true);
+    phase_poly(ctx, targetReg, 0, -0.5 * s, 0);
+    ctx_ft(ctx, targetReg, false);
+    phase_poly(ctx, targetReg, 0, 0.5 * (ch - 1) / s, 0);
+    phase_poly(ctx, targetReg, 0, -0.5 * sh / s, 0, ctrlReg, ctrlQubit);
+    ctx_ft(ctx, targetReg, true);
+    CVDV_CATCH()
+}
+
+void cvdvBeamSplitter(CVDVContext* ctx, int reg1, int reg2, double theta) {
+    // gates.cu:461-502: half turns by round (half away from zero)
+    if (!ready(ctx) || !two_regs_ok(ctx, reg1, reg2)) return;
+    if (reg1 == reg2) {
+        cvdv_set_error(CVDV_ERR_ARG, "Beam splitter requires two different registers");
+        return;
+    }
+    CVDV_TRY
+    const double theta0 = std::round(theta / PI) * PI;
+    const double rem = theta - theta0;
+    int ht = (int)std::round(theta0 / PI);
+    ht = ((ht % 4) + 4) % 4;
+    if ((ht == 1 || ht == 3) && ctx->nq[reg1] != ctx->nq[reg2]) {
+        cvdv_set_error(CVDV_ERR_ARG, "SWAP requires registers with same number of qubits");
+        return;
+    }
+    if (ht == 1) {
+        ctx_ft(ctx, reg1, false);
+        ctx_ft(ctx, reg2, false);
+        swap_regs(ctx, reg1, reg2);
+    } else if (ht == 2) {
+        parity(ctx, reg1);
+        parity(ctx, reg2);
+    } else if (ht == 3) {
+        ctx_ft(ctx, reg1, true);
+        ctx_ft(ctx, reg2, true);
+        swap_regs(ctx, reg1, reg2);
+    }
+    bs_small(ctx, reg1, reg2, rem);
+    CVDV_CATCH()
+}
+
+void cvdvConditionalBeamSplitter(CVDVContext* ctx, int reg1, int reg2, int ctrlReg, int ctrlQubit,
+                                 double theta) {
+    // gates.cu:504-547
+    if (!ready(ctx) || !two_regs_ok(ctx, reg1, reg2) || !ctrl_ok(ctx, ctrlReg, ctrlQubit)) return;
+    if (reg1 == reg2) {
+        cvdv_set_error(CVDV_ERR_ARG, "Conditional beam splitter requires two different registers");
+        return;
+    }
+    CVDV_TRY
+    const double theta0 = std::round(theta / PI) * PI;
+    const double rem = theta - theta0;
+    int ht = (int)std::round(theta0 / PI);
+    ht = ((ht % 4) + 4) % 4;
+    if ((ht == 1 || ht == 3) && ctx->nq[reg1] != ctx->nq[reg2]) {
+        cvdv_set_error(CVDV_ERR_ARG, "SWAP requires registers with same number of qubits");
+        return;
+    }
+    if (ht == 1 || ht == 3) {
+        ctx_ft(ctx, reg1, ht == 3);
+        ctx_ft(ctx, reg2, ht == 3);
+        cparity(ctx, reg1, ctrlReg, ctrlQubit);
+        cparity(ctx, reg2, ctrlReg, ctrlQubit);
+        swap_regs(ctx, reg1, reg2);
+    } else if (ht == 2) {
+        parity(ctx, reg1);
+        parity(ctx, reg2);
+    }
+    bs_small(ctx, reg1, reg2, rem, ctrlReg, ctrlQubit);
+    CVDV_CATCH()
+}
+
+void cvdvQ1Q2Gate(CVDVContext* ctx, int reg1, int reg2, double coeff) {
+    // gates.cu:549-565: e^{i coeff q1 q2}
+    if (!ready(ctx) || !two_regs_ok(ctx, reg1, reg2)) return;
+    if (reg1 == reg2) {
+        cvdv_set_error(CVDV_ERR_ARG, "Cannot apply Q1Q2 gate to the same register");
+        return;
+    }
+    CVDV_TRY phase_bilin(ctx, reg1, reg2, coeff); CVDV_CATCH()
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+"""hipcvdv - MI355X-native hybrid CV-DV state-vector engine (drop-in for CudaCvdv)."""
+
+from .hipCvdv import HipCvdv
+from .separable import SeparableState
+
+__all__ = ["HipCvdv", "SeparableState"]

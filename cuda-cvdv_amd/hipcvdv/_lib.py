@@ -1,0 +1,112 @@
+"""ctypes binding of libcvdv_hip.so (the C ABI declared in include/cvdv_hip.h).
+
+The argument types mirror the reference binding in src/cudaCvdv.py:35-73, so
+the same calls work on either library.  The library must have been built
+(``python -c "import __graft_entry__ as g; g.build()"`` or ``make -C
+cuda-cvdv_amd/csrc``); there is no fallback: a missing library raises.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_double, c_int, c_size_t, c_void_p
+from typing import Optional
+
+LIB_NAME = "libcvdv_hip.so"
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), LIB_NAME)
+
+_lib: Optional[ctypes.CDLL] = None
+
+# name -> (argtypes, restype)
+SIGNATURES = {
+    "cvdvCreate": ([c_int, POINTER(c_int)], c_void_p),
+    "cvdvCreateEx": ([c_int, POINTER(c_int), c_int], c_void_p),
+    "cvdvDestroy": ([c_void_p], None),
+    "cvdvFree": ([c_void_p], None),
+    "cvdvInitFromSeparable": ([c_void_p, POINTER(c_void_p), c_int], None),
+    "cvdvSetStateFromDevicePtr": ([c_void_p, c_void_p], None),
+    "cvdvGetNumRegisters": ([c_void_p], c_int),
+    "cvdvGetTotalSize": ([c_void_p], c_size_t),
+    "cvdvGetRegisterInfo": ([c_void_p, POINTER(c_int), POINTER(c_double)], None),
+    "cvdvGetRegisterDim": ([c_void_p, c_int], c_int),
+    "cvdvGetRegisterDx": ([c_void_p, c_int], c_double),
+    "cvdvFtQ2P": ([c_void_p, c_int], None),
+    "cvdvFtP2Q": ([c_void_p, c_int], None),
+    "cvdvDisplacement": ([c_void_p, c_int, c_double, c_double], None),
+    "cvdvConditionalDisplacement": ([c_void_p, c_int, c_int, c_int, c_double, c_double], None),
+    "cvdvPauliRotation": ([c_void_p, c_int, c_int, c_int, c_double], None),
+    "cvdvHadamard": ([c_void_p, c_int, c_int], None),
+    "cvdvParity": ([c_void_p, c_int], None),
+    "cvdvConditionalParity": ([c_void_p, c_int, c_int, c_int], None),
+    "cvdvSwapRegisters": ([c_void_p, c_int, c_int], None),
+    "cvdvPhaseSquare": ([c_void_p, c_int, c_double], None),
+    "cvdvPhaseCubic": ([c_void_p, c_int, c_double], None),
+    "cvdvRotation": ([c_void_p, c_int, c_double], None),
+    "cvdvConditionalRotation": ([c_void_p, c_int, c_int, c_int, c_double], None),
+    "cvdvSqueeze": ([c_void_p, c_int, c_double], None),
+    "cvdvConditionalSqueeze": ([c_void_p, c_int, c_int, c_int, c_double], None),
+    "cvdvBeamSplitter": ([c_void_p, c_int, c_int, c_double], None),
+    "cvdvConditionalBeamSplitter": ([c_void_p, c_int, c_int, c_int, c_int, c_double], None),
+    "cvdvQ1Q2Gate": ([c_void_p, c_int, c_int, c_double], None),
+    "cvdvGetWigner": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvGetHusimiQ": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvGetHusimiQOverlap": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvGetHusimiQWigner": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvMeasureMultiple": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
+    "cvdvMeasureMultipleCT": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
+    "cvdvGetState": ([c_void_p, POINTER(c_double), POINTER(c_double)], None),
+    "cvdvGetNorm": ([c_void_p], c_double),
+    "cvdvGetFidelity": ([c_void_p, POINTER(c_void_p), c_int, POINTER(c_double)], None),
+    "cvdvGetPhotonNumber": ([c_void_p, c_int], c_double),
+    "cvdvFidelityStatevectors": ([c_void_p, c_void_p, POINTER(c_double)], None),
+    "cvdvGetPrecision": ([c_void_p], c_int),
+    "cvdvGetLastError": ([ctypes.c_char_p, c_size_t], c_int),
+    "cvdvSynchronize": ([c_void_p], None),
+    "cvdvFlush": ([c_void_p], None),
+    "cvdvGetStateDevice": ([c_void_p, c_void_p], None),
+}
+
+
+def load(path: str = LIB_PATH) -> ctypes.CDLL:
+    """Load the library and attach argtypes; raises if it has not been built."""
+    if not os.path.exists(path):
+        raise RuntimeError(
+            f"{LIB_NAME} not found at {path}; build it first "
+            "(make -C cuda-cvdv_amd/csrc, or __graft_entry__.build())")
+    lib = ctypes.CDLL(path)
+    for name, (argt, rest) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argt
+        fn.restype = rest
+    return lib
+
+
+def get_lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        _lib = load()
+    return _lib
+
+
+class CvdvError(RuntimeError):
+    """A failure recorded by libcvdv_hip (see cvdvGetLastError)."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libcvdv_hip error {code}: {msg}")
+        self.code = code
+
+
+ERR_ARG = 1
+
+
+def check() -> None:
+    """Raise the last recorded library error, if any (ValueError for bad arguments)."""
+    lib = get_lib()
+    buf = ctypes.create_string_buffer(512)
+    code = lib.cvdvGetLastError(buf, 512)
+    if code:
+        msg = buf.value.decode(errors="replace")
+        if code == ERR_ARG:
+            raise ValueError(msg)
+        raise CvdvError(code, msg)

@@ -1,0 +1,105 @@
+/* cvdv_hip.h - C ABI of libcvdv_hip.so, the MI355X (gfx950) hybrid CV-DV
+ * state-vector engine.
+ *
+ * Every function named cvdv* below except those marked "addition" has the
+ * same name, argument list and meaning as the reference's C ABI in
+ * src/cuda/api.h (line cited per entry), so a caller of libcvdv.so (e.g.
+ * src/cudaCvdv.py:35-73) can load this library instead without change.
+ *
+ * Conventions (identical to the reference): register 0 is the most
+ * significant part of the flat index; qubit k of register r is bit
+ * (n_r - 1 - k) of that register's local index; controlled gates apply +phase
+ * when the control bit is 0 and -phase when it is 1; grid x_j = (j-(N-1)/2)dx,
+ * dx = sqrt(2 pi / N).  Device pointers passed in hold complex128 data.
+ *
+ * Errors: invalid arguments print a message to stderr and leave the state
+ * unchanged, as the reference does; in addition every failure (including HIP
+ * runtime failures, which the reference turns into C++ exceptions) is recorded
+ * and can be read with cvdvGetLastError.  No C++ exception crosses this ABI.
+ */
+#ifndef CVDV_HIP_H
+#define CVDV_HIP_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct CVDVContext CVDVContext;
+
+/* ---- lifecycle and state (api.h:14-23) ---- */
+CVDVContext* cvdvCreate(int numReg, int* numQubits);                 /* api.h:14 */
+void   cvdvDestroy(CVDVContext* ctx);                                /* api.h:15 */
+void   cvdvFree(CVDVContext* ctx);                                   /* api.h:16 */
+void   cvdvInitFromSeparable(CVDVContext* ctx, void** devicePtrs, int numReg); /* api.h:17 */
+void   cvdvSetStateFromDevicePtr(CVDVContext* ctx, void* d_src);     /* api.h:18 */
+int    cvdvGetNumRegisters(CVDVContext* ctx);                        /* api.h:19 */
+size_t cvdvGetTotalSize(CVDVContext* ctx);                           /* api.h:20 */
+void   cvdvGetRegisterInfo(CVDVContext* ctx, int* qubitCountsOut, double* gridStepsOut); /* api.h:21 */
+int    cvdvGetRegisterDim(CVDVContext* ctx, int regIdx);             /* api.h:22 */
+double cvdvGetRegisterDx(CVDVContext* ctx, int regIdx);              /* api.h:23 */
+
+/* ---- gates (api.h:25-46) ---- */
+void cvdvFtQ2P(CVDVContext* ctx, int regIdx);                        /* api.h:25 */
+void cvdvFtP2Q(CVDVContext* ctx, int regIdx);                        /* api.h:26 */
+void cvdvDisplacement(CVDVContext* ctx, int regIdx, double betaRe, double betaIm); /* api.h:27 */
+void cvdvConditionalDisplacement(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit,
+                                 double betaRe, double betaIm);      /* api.h:28-29 */
+void cvdvPauliRotation(CVDVContext* ctx, int regIdx, int qubitIdx, int axis, double theta); /* api.h:30 */
+void cvdvHadamard(CVDVContext* ctx, int regIdx, int qubitIdx);       /* api.h:31 */
+void cvdvParity(CVDVContext* ctx, int regIdx);                       /* api.h:32 */
+void cvdvConditionalParity(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit); /* api.h:33 */
+void cvdvSwapRegisters(CVDVContext* ctx, int reg1, int reg2);        /* api.h:34 */
+void cvdvPhaseSquare(CVDVContext* ctx, int regIdx, double t);        /* api.h:35 */
+void cvdvPhaseCubic(CVDVContext* ctx, int regIdx, double t);         /* api.h:36 */
+void cvdvRotation(CVDVContext* ctx, int regIdx, double theta);       /* api.h:37 */
+void cvdvConditionalRotation(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit,
+                             double theta);                          /* api.h:38-39 */
+void cvdvSqueeze(CVDVContext* ctx, int regIdx, double r);            /* api.h:40 */
+void cvdvConditionalSqueeze(CVDVContext* ctx, int targetReg, int ctrlReg, int ctrlQubit,
+                            double r);                               /* api.h:41-42 */
+void cvdvBeamSplitter(CVDVContext* ctx, int reg1, int reg2, double theta); /* api.h:43 */
+void cvdvConditionalBeamSplitter(CVDVContext* ctx, int reg1, int reg2, int ctrlReg,
+                                 int ctrlQubit, double theta);       /* api.h:44-45 */
+void cvdvQ1Q2Gate(CVDVContext* ctx, int reg1, int reg2, double coeff); /* api.h:46 */
+
+/* ---- phase-space readout (api.h:48-51); outputs are row-major N x N doubles,
+ *      rows = momentum, columns = position ---- */
+void cvdvGetWigner(CVDVContext* ctx, int regIdx, double* wignerOut);   /* api.h:48 */
+void cvdvGetHusimiQ(CVDVContext* ctx, int regIdx, double* husimiOut);  /* api.h:49 */
+void cvdvGetHusimiQOverlap(CVDVContext* ctx, int regIdx, double* husimiOut); /* api.h:50 */
+void cvdvGetHusimiQWigner(CVDVContext* ctx, int regIdx, double* husimiOut);  /* api.h:51 */
+
+/* ---- measurement and state readout (api.h:53-61) ---- */
+/* probsOut holds prod(dim) doubles, column-major: first selected register fastest. */
+void   cvdvMeasureMultiple(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut); /* api.h:53-54 */
+/* The reference's cuTENSOR comparator (api.h:55-56); here the same deterministic
+ * reduction as cvdvMeasureMultiple, kept for ABI compatibility. */
+void   cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut);
+void   cvdvGetState(CVDVContext* ctx, double* realOut, double* imagOut);    /* api.h:57 */
+double cvdvGetNorm(CVDVContext* ctx);                                      /* api.h:58 */
+void   cvdvGetFidelity(CVDVContext* ctx, void** devicePtrs, int numReg, double* fidOut); /* api.h:59 */
+double cvdvGetPhotonNumber(CVDVContext* ctx, int regIdx);                  /* api.h:60 */
+void   cvdvFidelityStatevectors(CVDVContext* ctx1, CVDVContext* ctx2, double* fidOut); /* api.h:61 */
+
+/* ---- additions (no counterpart in api.h) ---- */
+/* Precision mode: 0 = complex128 (as cvdvCreate), 1 = mixed (complex64
+ * amplitudes; FP64 phase angles with range reduction, FP64 twiddle
+ * generation, FP64 compensated probability/norm sums). */
+CVDVContext* cvdvCreateEx(int numReg, int* numQubits, int precision);
+int    cvdvGetPrecision(CVDVContext* ctx);
+/* Returns the last error code (0 = none) of this thread and copies its message
+ * (truncated to len) into buf when buf is not NULL; clears it. */
+int    cvdvGetLastError(char* buf, size_t len);
+/* Blocks until every operation issued on the context has finished. */
+void   cvdvSynchronize(CVDVContext* ctx);
+/* Applies any deferred diagonal phases and restores the canonical register
+ * layout (both are otherwise applied lazily by the next kernel). */
+void   cvdvFlush(CVDVContext* ctx);
+/* Copies the state (canonical layout, complex128) to a caller device buffer. */
+void   cvdvGetStateDevice(CVDVContext* ctx, void* d_dst);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CVDV_HIP_H */

@@ -69,8 +69,44 @@ void* ctx_scratch(CVDVContext* ctx, size_t bytes) {
     return ctx->scratch;
 }
 
+// ---------------------------------------------------------------- profiling
+static hipEvent_t take_event(CVDVContext* ctx) {
+    if (!ctx->evPool.empty()) {
+        hipEvent_t e = ctx->evPool.back();
+        ctx->evPool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    return e;
+}
+
+int prof_begin(CVDVContext* ctx, int kind) {
+    if (!ctx->profOn) return -1;
+    ProfRec r{kind, take_event(ctx), take_event(ctx)};
+    HIPCHK(hipEventRecord(r.a, ctx->stream));
+    ctx->prof.push_back(r);
+    return (int)ctx->prof.size() - 1;
+}
+
+void prof_end(CVDVContext* ctx, int idx) {
+    if (idx < 0) return;
+    HIPCHK(hipEventRecord(ctx->prof[idx].b, ctx->stream));
+}
+
+static void prof_clear(CVDVContext* ctx) {
+    for (auto& r : ctx->prof) {
+        ctx->evPool.push_back(r.a);
+        ctx->evPool.push_back(r.b);
+    }
+    ctx->prof.clear();
+}
+
 static void ctx_release(CVDVContext* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    prof_clear(ctx);
+    for (hipEvent_t e : ctx->evPool) (void)hipEventDestroy(e);
+    ctx->evPool.clear();
     for (int i = 0; i < 2; ++i)
         if (ctx->buf[i]) (void)hipFree(ctx->buf[i]);
     ctx->buf[0] = ctx->buf[1] = nullptr;
@@ -192,6 +228,7 @@ void ctx_materialize(CVDVContext* ctx) {
     DiagProg p = ctx_take_pending(ctx);
     void* dst = ctx->other();
     const uint64_t n = ctx->totalSize;
+    const int pid = prof_begin(ctx, KK_OTHER);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_materialize<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            (double2*)dst, (const double2*)ctx->state(), n, L, p);
@@ -199,6 +236,7 @@ void ctx_materialize(CVDVContext* ctx) {
         hipLaunchKernelGGL(k_materialize<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            (float2*)dst, (const float2*)ctx->state(), n, L, p);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
     ctx->cur = 1 - ctx->cur;
     for (int r = 0; r < ctx->numReg; ++r) {
         ctx->pshift[r] = ctx->flw[r];
@@ -384,6 +422,43 @@ int cvdvGetRegisterDim(CVDVContext* ctx, int regIdx) {
 double cvdvGetRegisterDx(CVDVContext* ctx, int regIdx) {
     if (!ctx || regIdx < 0 || regIdx >= ctx->numReg) return -1.0;
     return ctx->dx[regIdx];
+}
+
+void cvdvProfileEnable(CVDVContext* ctx, int on) {
+    if (!ctx) return;
+    CVDV_TRY
+    if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_clear(ctx);
+    ctx->profOn = on != 0;
+    CVDV_CATCH()
+}
+
+int cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs) {
+    if (!ctx) return 0;
+    CVDV_TRY
+    if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+    int n = 0;
+    double tot = 0.0, mx = 0.0;
+    for (auto& r : ctx->prof) {
+        if (r.kind != kind) continue;
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
+        tot += ms;
+        mx = ms > mx ? ms : mx;
+        ++n;
+    }
+    if (totalMs) *totalMs = tot;
+    if (maxMs) *maxMs = mx;
+    return n;
+    CVDV_CATCH(0)
+}
+
+void cvdvProfileReset(CVDVContext* ctx) {
+    if (!ctx) return;
+    CVDV_TRY
+    if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_clear(ctx);
+    CVDV_CATCH()
 }
 
 void cvdvFlush(CVDVContext* ctx) {

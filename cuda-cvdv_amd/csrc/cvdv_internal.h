@@ -120,7 +120,11 @@ __device__ __forceinline__ double2 unit_rev(double f, double) {
 __device__ __forceinline__ float2 unit_rev(double f, float) {
     float s, c;
     sincospif((float)(2.0 * f), &s, &c);
-    return make_float2(c, s);
+    // Pull |e|^2 back to 1 (to first order, exactly in FP64) before the single
+    // rounding to FP32, so repeated phase passes leave the norm unbiased.
+    const double r2 = (double)c * c + (double)s * s;
+    const double k = 1.5 - 0.5 * r2;
+    return make_float2((float)(c * k), (float)(s * k));
 }
 
 template <typename C>
@@ -131,6 +135,15 @@ __device__ __forceinline__ C apply_diag(const DiagProg& p, uint64_t g, C v) {
 
 // ---------------------------------------------------------------- context
 struct TwiddleCache;  // fft.hip
+
+// Kernel kinds for the optional per-launch event timing (cvdvProfileEnable).
+enum CvdvKernelKind { KK_FFT = 0, KK_DIAG = 1, KK_QUBIT = 2, KK_CPARITY = 3, KK_REDUCE = 4,
+                      KK_OTHER = 5, KK_COUNT = 6 };
+
+struct ProfRec {
+    int kind;
+    hipEvent_t a, b;
+};
 
 struct CVDVContext {
     int numReg = 0;
@@ -150,6 +163,9 @@ struct CVDVContext {
     TwiddleCache* tw = nullptr;
     void* scratch = nullptr;          // readout scratch (grown on demand)
     size_t scratchBytes = 0;
+    bool profOn = false;
+    std::vector<ProfRec> prof;        // recorded launches
+    std::vector<hipEvent_t> evPool;   // reusable events
 
     size_t elemBytes() const { return prec == CVDV_C128 ? 16 : 8; }
     void* state() const { return buf[cur]; }
@@ -189,6 +205,10 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
 
 // scratch
 void* ctx_scratch(CVDVContext* ctx, size_t bytes);
+
+// profiling (ctx.hip): bracket a launch on ctx->stream when profiling is on
+int prof_begin(CVDVContext* ctx, int kind);
+void prof_end(CVDVContext* ctx, int idx);
 
 inline uint64_t ctx_ctrl_bit(const CVDVContext* c, int reg, int qubit) {
     return (uint64_t)(c->pshift[reg] + c->nq[reg] - 1 - qubit);

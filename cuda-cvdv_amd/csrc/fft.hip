@@ -73,7 +73,39 @@ static StagePlan stage_plan(int logM) {
     return p;
 }
 
-static void* upload_table(const std::vector<double>& re, const std::vector<double>& im, int prec) {
+// FP32 twiddles: every entry is one of the floats adjacent to (or equal to) the
+// exact value, chosen by error diffusion so that sum(|w|^2 - 1) over each
+// stage stays near zero.  Rounding each entry to nearest leaves a mean
+// |w|^2 - 1 of order -1e-8, which shrinks the norm by ~2e-8 per FFT.
+static void round_unit_diffused(const double* re, const double* im, size_t n, float2* out) {
+    double acc = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        const float c0 = (float)re[i], s0 = (float)im[i];
+        const float cs[3] = {nextafterf(c0, -2.f), c0, nextafterf(c0, 2.f)};
+        const float ss[3] = {nextafterf(s0, -2.f), s0, nextafterf(s0, 2.f)};
+        double bestErr = 0.0, bestScore = 1e300;
+        float bc = c0, bs = s0;
+        for (int a = 0; a < 3; ++a) {
+            if (std::fabs((double)cs[a] - re[i]) > std::fabs((double)nextafterf(c0, 2.f) - (double)c0)) continue;
+            for (int b = 0; b < 3; ++b) {
+                if (std::fabs((double)ss[b] - im[i]) > std::fabs((double)nextafterf(s0, 2.f) - (double)s0)) continue;
+                const double err = (double)cs[a] * cs[a] + (double)ss[b] * ss[b] - 1.0;
+                const double score = std::fabs(acc + err);
+                if (score < bestScore) {
+                    bestScore = score;
+                    bestErr = err;
+                    bc = cs[a];
+                    bs = ss[b];
+                }
+            }
+        }
+        acc += bestErr;
+        out[i] = make_float2(bc, bs);
+    }
+}
+
+static void* upload_table(const std::vector<double>& re, const std::vector<double>& im, int prec,
+                          const std::vector<size_t>& segments = {}) {
     const size_t n = re.size();
     void* d = nullptr;
     if (prec == CVDV_C128) {
@@ -83,7 +115,13 @@ static void* upload_table(const std::vector<double>& re, const std::vector<doubl
         HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
     } else {
         std::vector<float2> h(n ? n : 1);
-        for (size_t i = 0; i < n; ++i) h[i] = make_float2((float)re[i], (float)im[i]);
+        size_t start = 0;
+        std::vector<size_t> seg = segments;
+        seg.push_back(n);
+        for (size_t end : seg) {
+            if (end > start) round_unit_diffused(re.data() + start, im.data() + start, end - start, h.data() + start);
+            start = end;
+        }
         HIPCHK(hipMalloc(&d, h.size() * sizeof(float2)));
         HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
@@ -111,21 +149,25 @@ static void* stage_table(CVDVContext* ctx, int logM, int prec) {
                 host_twiddle(e, M, re[p.off[s] + j * (R - 1) + i - 1], im[p.off[s] + j * (R - 1) + i - 1]);
             }
     }
-    void* d = upload_table(re, im, prec);
+    std::vector<size_t> segs;
+    for (int st = 1; st < p.nst; ++st) segs.push_back((size_t)p.off[st]);
+    void* d = upload_table(re, im, prec, segs);
     tc->stage[{logM, prec}] = d;
     return d;
 }
 
-static void* full_table(CVDVContext* ctx, int logN, int prec) {
+// W_N^e for e in [0, N), always FP64 (the four-step twiddle and the direct DFT
+// apply it in FP64 for both precisions).
+static const double2* full_table(CVDVContext* ctx, int logN) {
     TwiddleCache* tc = cache(ctx);
-    auto it = tc->full.find({logN, prec});
-    if (it != tc->full.end()) return it->second;
+    auto it = tc->full.find({logN, CVDV_C128});
+    if (it != tc->full.end()) return (const double2*)it->second;
     const uint64_t N = 1ull << logN;
     std::vector<double> re(N), im(N);
     for (uint64_t e = 0; e < N; ++e) host_twiddle(e, N, re[e], im[e]);
-    void* d = upload_table(re, im, prec);
-    tc->full[{logN, prec}] = d;
-    return d;
+    void* d = upload_table(re, im, CVDV_C128);
+    tc->full[{logN, CVDV_C128}] = d;
+    return (const double2*)d;
 }
 
 // ---------------------------------------------------------------- pass descriptor
@@ -138,7 +180,7 @@ struct FftPass {
     uint64_t nLo, nH2, nH1;
     uint64_t inH1, inH2, inR;
     uint64_t outH1, outH2, outR;
-    double outScale;
+    double outScale;  // applied as hi + lo parts (FMA) so an FP32 store keeps it unbiased
 };
 
 // ---------------------------------------------------------------- butterflies
@@ -174,22 +216,30 @@ __device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3) {
     x3 = csub(t1, t3);
 }
 
+// v / sqrt(2).  In FP32 the constant is carried as hi + lo and applied with an
+// FMA: a single rounded constant is 1.1e-8 low, which would shrink the norm by
+// ~3e-8 on every radix-8 stage instead of leaving an unbiased rounding error.
+__device__ __forceinline__ double mul_rsqrt2(double v) { return v * 0.70710678118654752440; }
+__device__ __forceinline__ float mul_rsqrt2(float v) {
+    const float hi = 0.707106769084930419921875f;  // nearest float below 1/sqrt(2)
+    const float lo = 1.2101617485882343e-08f;      // 1/sqrt(2) - hi
+    return fmaf(v, lo, v * hi);
+}
+
 template <bool INV, typename C>
 __device__ __forceinline__ void dft8(C* x) {
-    using R = decltype(x[0].x);
     C a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6];
     C b0 = x[1], b1 = x[3], b2 = x[5], b3 = x[7];
     dft4<INV>(a0, a1, a2, a3);
     dft4<INV>(b0, b1, b2, b3);
-    const R h = (R)0.70710678118654752440;
     // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = (-1 - i)/sqrt2 (forward)
     C w1b, w3b;
     if (!INV) {
-        w1b.x = h * (b1.x + b1.y); w1b.y = h * (b1.y - b1.x);
-        w3b.x = h * (b3.y - b3.x); w3b.y = -h * (b3.x + b3.y);
+        w1b.x = mul_rsqrt2(b1.x + b1.y); w1b.y = mul_rsqrt2(b1.y - b1.x);
+        w3b.x = mul_rsqrt2(b3.y - b3.x); w3b.y = -mul_rsqrt2(b3.x + b3.y);
     } else {
-        w1b.x = h * (b1.x - b1.y); w1b.y = h * (b1.y + b1.x);
-        w3b.x = -h * (b3.x + b3.y); w3b.y = h * (b3.x - b3.y);
+        w1b.x = mul_rsqrt2(b1.x - b1.y); w1b.y = mul_rsqrt2(b1.y + b1.x);
+        w3b.x = -mul_rsqrt2(b3.x + b3.y); w3b.y = mul_rsqrt2(b3.x - b3.y);
     }
     const C w2b = rot_mi<INV>(b2);
     x[0] = cadd(a0, b0); x[4] = csub(a0, b0);
@@ -247,7 +297,7 @@ __device__ __forceinline__ void stockham_stage(C* lds, const FftPass& d, int s, 
 template <typename R, bool INV>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, DiagProg pre,
-           const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
+           const typename CT<R>::type* __restrict__ tw, const double2* __restrict__ tw4) {
     using C = typename CT<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
@@ -285,6 +335,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     }
 
     const R scale = (R)d.outScale;
+    const R scaleLo = (R)(d.outScale - (double)scale);
     const uint64_t nmask = (1ull << d.tw4LogN) - 1ull;
     for (int e = tid; e < tile; e += nt) {
         const int c = e % Cc;
@@ -294,10 +345,12 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         C v = lds[(hb * M + row) * Cc + c];
         if (d.tw4) {
             const uint64_t j1 = (lo0 + c) >> d.tw4Shift;
-            v = mul_w<INV>(v, tw4[(j1 * (uint64_t)row) & nmask]);
+            const double2 w = mul_w<INV>(make_double2((double)v.x, (double)v.y),
+                                         tw4[(j1 * (uint64_t)row) & nmask]);
+            v = cmk<C>(w.x, w.y);
         }
-        v.x *= scale;
-        v.y *= scale;
+        v.x = fma(v.x, scaleLo, v.x * scale);
+        v.y = fma(v.y, scaleLo, v.y * scale);
         out[outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c] = v;
     }
 }
@@ -306,7 +359,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
 template <typename R, bool INV>
 __global__ void k_dft_naive(const typename CT<R>::type* in, typename CT<R>::type* out, uint64_t total,
                             int sh, int nb, DiagProg pre, int usePre, double scale,
-                            const typename CT<R>::type* __restrict__ twN) {
+                            const double2* __restrict__ twN) {
     using C = typename CT<R>::type;
     const uint64_t M = 1ull << nb, fmask = (M - 1) << sh;
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
@@ -318,12 +371,11 @@ __global__ void k_dft_naive(const typename CT<R>::type* in, typename CT<R>::type
             const uint64_t gi = base | (j << sh);
             C v = in[gi];
             if (usePre) v = apply_diag(pre, gi, v);
-            const C w = twN[(j * k) & (M - 1)];
-            const C p = mul_w<INV>(v, w);
-            ax += (double)p.x;
-            ay += (double)p.y;
+            const double2 p = mul_w<INV>(make_double2((double)v.x, (double)v.y), twN[(j * k) & (M - 1)]);
+            ax += p.x;
+            ay += p.y;
         }
-        out[g] = cmk<C>(ax * scale, ay * scale);
+        out[g] = cmk<C>(ax * scale, ay * scale);  // accumulated in FP64, rounded once
     }
 }
 
@@ -371,11 +423,12 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
         d.off[s] = sp.off[s];
     }
     const C* tw = (const C*)stage_table(ctx, __builtin_ctz((unsigned)d.M), prec);
-    const C* tw4 = d.tw4 ? (const C*)full_table(ctx, d.tw4LogN, prec) : tw;
+    const double2* tw4 = d.tw4 ? full_table(ctx, d.tw4LogN) : nullptr;
     const int tile = d.M * d.C * d.Hb;
     const int nt = tile / 8;
     const uint64_t blocks = d.nH1 * (d.nH2 / d.Hb) * (d.nLo / d.C);
     const size_t lds = (size_t)tile * sizeof(C);
+    const int pid = prof_begin(ctx, KK_FFT);
     if (inverse)
         hipLaunchKernelGGL((k_fft_tile<R, true>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
                            in, out, d, pre, tw, tw4);
@@ -383,6 +436,7 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
         hipLaunchKernelGGL((k_fft_tile<R, false>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
                            in, out, d, pre, tw, tw4);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
 }
 
 // Runs the FFT over field [sh, sh+nb) of a buffer of `total` elements.
@@ -408,9 +462,10 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     }
     if (total < 4096 || N <= 64) {
         // tiny state: direct DFT, out of place
-        const C* twN = (const C*)full_table(ctx, nb, prec);
+        const double2* twN = full_table(ctx, nb);
         C* tmp = tmpFn();
         uint64_t blocks = (total + 255) / 256;
+        const int pid = prof_begin(ctx, KK_FFT);
         if (inverse)
             hipLaunchKernelGGL((k_dft_naive<R, true>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                                data, tmp, total, sh, nb, pre, usePre, outScale, twN);
@@ -418,6 +473,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
             hipLaunchKernelGGL((k_dft_naive<R, false>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
                                data, tmp, total, sh, nb, pre, usePre, outScale, twN);
         HIPCHK(hipGetLastError());
+        prof_end(ctx, pid);
         return true;
     }
     // four-step: j = j1 + N1 j2

@@ -26,9 +26,11 @@ __global__ void k_diag(C* __restrict__ s, uint64_t total, DiagProg p) {
 }
 
 // 2x2 unitary on the pair (bit = 0, bit = 1); pending phases applied on load.
+// The matrix product runs in FP64 for both precisions, so a complex64 state is
+// rounded once per gate and never sees a rounded (non-unitary) matrix.
 template <typename C>
-__global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, C m00, C m01, C m10, C m11,
-                        DiagProg p, int usep) {
+__global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, double2 m00, double2 m01,
+                        double2 m10, double2 m11, DiagProg p, int usep) {
     const uint64_t lowmask = (1ull << bit) - 1ull;
     for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
          pid += (uint64_t)gridDim.x * blockDim.x) {
@@ -39,12 +41,11 @@ __global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, C m00, C m0
             a = apply_diag(p, i0, a);
             b = apply_diag(p, i1, b);
         }
-        C na = cmul(m00, a), nb = cmul(m10, a);
-        const C t0 = cmul(m01, b), t1 = cmul(m11, b);
-        na.x += t0.x; na.y += t0.y;
-        nb.x += t1.x; nb.y += t1.y;
-        s[i0] = na;
-        s[i1] = nb;
+        const double2 ad = make_double2((double)a.x, (double)a.y);
+        const double2 bd = make_double2((double)b.x, (double)b.y);
+        const double2 t00 = cmul(m00, ad), t01 = cmul(m01, bd), t10 = cmul(m10, ad), t11 = cmul(m11, bd);
+        s[i0] = cmk<C>(t00.x + t01.x, t00.y + t01.y);
+        s[i1] = cmk<C>(t10.x + t11.x, t10.y + t11.y);
     }
 }
 
@@ -95,6 +96,7 @@ DiagProg ctx_take_pending(CVDVContext* ctx) {
 
 void launch_diag(CVDVContext* ctx, const DiagProg& p) {
     const uint64_t n = ctx->totalSize;
+    const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_diag<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), n, p);
@@ -102,6 +104,7 @@ void launch_diag(CVDVContext* ctx, const DiagProg& p) {
         hipLaunchKernelGGL(k_diag<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
                            (float2*)ctx->state(), n, p);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
 }
 
 void ctx_flush(CVDVContext* ctx) {
@@ -233,17 +236,17 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
     DiagProg p = ctx_take_pending(ctx);
     const int usep = prog_active(p) ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
+    const double2 m00 = make_double2(mm[0], mm[1]), m01 = make_double2(mm[2], mm[3]);
+    const double2 m10 = make_double2(mm[4], mm[5]), m11 = make_double2(mm[6], mm[7]);
+    const int pid = prof_begin(ctx, KK_QUBIT);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_qubit<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
-                           (double2*)ctx->state(), np, bit, make_double2(mm[0], mm[1]),
-                           make_double2(mm[2], mm[3]), make_double2(mm[4], mm[5]),
-                           make_double2(mm[6], mm[7]), p, usep);
+                           (double2*)ctx->state(), np, bit, m00, m01, m10, m11, p, usep);
     else
         hipLaunchKernelGGL(k_qubit<float2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
-                           (float2*)ctx->state(), np, bit, make_float2(mm[0], mm[1]),
-                           make_float2(mm[2], mm[3]), make_float2(mm[4], mm[5]),
-                           make_float2(mm[6], mm[7]), p, usep);
+                           (float2*)ctx->state(), np, bit, m00, m01, m10, m11, p, usep);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
 }
 
 static void zbit(CVDVContext* ctx, int reg, int qubit, double theta) {
@@ -288,6 +291,7 @@ static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
     const int usep = prog_active(p) ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const int cpos = (int)ctx_ctrl_bit(ctx, creg, cq);
+    const int pid = prof_begin(ctx, KK_CPARITY);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_cparity<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
@@ -297,6 +301,7 @@ static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
                            (float2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
                            ctx->refl[tgt], p, usep);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
 }
 
 static void swap_regs(CVDVContext* ctx, int r1, int r2) {

@@ -167,6 +167,7 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
     double* fin = part + usplit * outSize;
     const unsigned blocks = (unsigned)(nGroups * usplit);
     const unsigned threads = 1u << TB;
+    const int pid = prof_begin(ctx, KK_REDUCE);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_marginal<double2>, dim3(blocks), dim3(threads), 0, ctx->stream,
                            (const double2*)ctx->state(), a, part);
@@ -174,6 +175,7 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
         hipLaunchKernelGGL(k_marginal<float2>, dim3(blocks), dim3(threads), 0, ctx->stream,
                            (const float2*)ctx->state(), a, part);
     HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
     const double* res = part;
     if (usplit > 1) {
         uint64_t b = (outSize + 255) / 256;

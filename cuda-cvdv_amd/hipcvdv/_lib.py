@@ -65,6 +65,9 @@ SIGNATURES = {
     "cvdvSynchronize": ([c_void_p], None),
     "cvdvFlush": ([c_void_p], None),
     "cvdvGetStateDevice": ([c_void_p, c_void_p], None),
+    "cvdvProfileEnable": ([c_void_p, c_int], None),
+    "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),
+    "cvdvProfileReset": ([c_void_p], None),
 }
 
 

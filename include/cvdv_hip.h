@@ -98,6 +98,13 @@ void   cvdvSynchronize(CVDVContext* ctx);
 void   cvdvFlush(CVDVContext* ctx);
 /* Copies the state (canonical layout, complex128) to a caller device buffer. */
 void   cvdvGetStateDevice(CVDVContext* ctx, void* d_dst);
+/* Per-launch timing with HIP events recorded on the context's stream.
+ * kind: 0 FFT passes, 1 diagonal passes, 2 qubit-pair passes, 3 conditional
+ * parity, 4 reductions, 5 layout/other.  Read returns the launch count and the
+ * summed (and largest) elapsed milliseconds since the last enable/reset. */
+void   cvdvProfileEnable(CVDVContext* ctx, int on);
+int    cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs);
+void   cvdvProfileReset(CVDVContext* ctx);
 
 #ifdef __cplusplus
 }

@@ -113,6 +113,9 @@ static void ctx_release(CVDVContext* ctx) {
     if (ctx->scratch) (void)hipFree(ctx->scratch);
     ctx->scratch = nullptr;
     ctx->scratchBytes = 0;
+    if (ctx->arena) (void)hipFree(ctx->arena);
+    ctx->arena = nullptr;
+    ctx->arenaBytes = 0;
     if (ctx->tw) twiddle_cache_free(ctx->tw);
     ctx->tw = nullptr;
     ctx->initialized = false;

@@ -163,6 +163,8 @@ struct CVDVContext {
     TwiddleCache* tw = nullptr;
     void* scratch = nullptr;          // readout scratch (grown on demand)
     size_t scratchBytes = 0;
+    void* arena = nullptr;            // phase tables of the compiled pending program
+    size_t arenaBytes = 0;
     bool profOn = false;
     std::vector<ProfRec> prof;        // recorded launches
     std::vector<hipEvent_t> evPool;   // reusable events
@@ -196,6 +198,8 @@ void launch_diag(CVDVContext* ctx, const DiagProg& p);
 // fft.hip
 void fft_register(CVDVContext* ctx, int pshift, int nbits, bool inverse, const DiagProg& pre,
                   double outScale);
+// largest dynamic LDS a workgroup may request on the current device
+size_t device_max_lds();
 void fft_generic_c128(CVDVContext* ctx, double2* data, double2* tmp, uint64_t total, int shift,
                       int nbits, bool inverse, double outScale);
 void twiddle_cache_free(TwiddleCache* tc);

@@ -7,18 +7,23 @@
 // C consecutive lo columns x M rows x Hb consecutive hi blocks in LDS, runs a
 // Stockham FFT there (radix 8/4/2 butterflies in registers, per-stage twiddle
 // tables generated in FP64), and writes the tile back:
-//   * one HBM pass when a tile of >= 128 contiguous bytes per row fits in LDS;
+//   * one HBM pass when the tile fits in LDS: C*M elements with C >= 16 (c64)
+//     or 8 (c128) columns for a strided register, or a whole contiguous
+//     register of up to 2^14 (c64) / 2^13 (c128) points in one workgroup;
 //   * otherwise the four-step split N = N1 * N2 (j = j1 + N1 j2):
-//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2};
+//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2} (FP64);
 //       pass 2 (out of place, transposing): FFT over j1, written at k2 + N2 k1.
-// Pending diagonal phases are applied as elements are loaded by the first
-// pass and the 1/sqrt(N) normalisation as they are stored by the last.
+// Pending diagonal phases are applied (compiled, see diag_fast.h) as elements
+// are loaded by the first pass and 1/sqrt(N) as they are stored by the last.
+// Every tile dimension is a power of two: indices use shifts and masks only.
+// complex64 tiles move 16 bytes per lane along the contiguous dimension.
 
 #include <cmath>
 #include <map>
 #include <vector>
 
 #include "cvdv_internal.h"
+#include "diag_fast.h"
 
 static const long double PI_L = 3.141592653589793238462643383279502884L;
 
@@ -31,7 +36,7 @@ struct TwKey {
 
 struct TwiddleCache {
     std::map<TwKey, void*> stage;  // per-stage Stockham tables for length M
-    std::map<TwKey, void*> full;   // W_N^e, e in [0, N)
+    std::map<TwKey, void*> full;   // W_N^e, e in [0, N), FP64
 };
 
 void twiddle_cache_free(TwiddleCache* tc) {
@@ -170,18 +175,34 @@ static const double2* full_table(CVDVContext* ctx, int logN) {
     return (const double2*)d;
 }
 
+size_t device_max_lds() {
+    static size_t cached = 0;
+    if (!cached) {
+        int dev = 0, v = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+        cached = (size_t)v;
+    }
+    return cached;
+}
+
 // ---------------------------------------------------------------- pass descriptor
 struct FftPass {
-    int M, C, Hb, rowFastIn, rowFastOut;
+    int lgM, lgC, lgHb;            // tile = [Hb][M][C]
+    int lgNLoBlk, lgNH2Blk;        // grid = nH1 * nH2Blk * nLoBlk
+    int rowFastIn, rowFastOut;     // linear order after the column index
+    int vecIn, vecOut;             // 1: pairs of complex64 move as one 16-byte access
     int nst;
-    int rad[6], L[6], off[6];
+    int rad[6], lgL[6], off[6];
     int tw4, tw4Shift, tw4LogN;
     int usePre;
-    uint64_t nLo, nH2, nH1;
     uint64_t inH1, inH2, inR;
     uint64_t outH1, outH2, outR;
     double outScale;  // applied as hi + lo parts (FMA) so an FP32 store keeps it unbiased
 };
+
+// LDS index with one pad slot per 32 entries (breaks power-of-two strides)
+__device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
 // ---------------------------------------------------------------- butterflies
 template <bool INV, typename C>
@@ -255,94 +276,122 @@ __device__ __forceinline__ void dft_r(C* x) {
     else dft8<INV>(x);
 }
 
-// One Stockham stage: each thread owns 8/RP butterflies of radix RP.
-template <int RP, bool INV, typename C>
+// One Stockham stage: each thread owns E/RP butterflies of radix RP.
+template <int RP, int E, bool INV, typename C>
 __device__ __forceinline__ void stockham_stage(C* lds, const FftPass& d, int s, const C* __restrict__ tw,
                                                int tid, int nt) {
-    constexpr int NB = 8 / RP;
-    const int M = d.M, Cc = d.C, L = d.L[s], Mq = M / RP;
-    C x[8];
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgM = d.lgM, lgC = d.lgC, lgL = d.lgL[s];
+    const int lgMq = lgM - LGR;
+    const int cmask = (1 << lgC) - 1, qmask = (1 << lgMq) - 1, lmask = (1 << lgL) - 1;
+    C x[E];
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const int b = tid + k * nt;
-        const int c = b % Cc;
-        const int rest = b / Cc;
-        const int q = rest % Mq;
-        const int base = (rest / Mq) * M;
+        const int c = b & cmask;
+        const int rest = b >> lgC;
+        const int q = rest & qmask;
+        const int base = (rest >> lgMq) << lgM;
 #pragma unroll
-        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[(base + q + i * Mq) * Cc + c];
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_ix(((base + q + (i << lgMq)) << lgC) + c)];
     }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const int b = tid + k * nt;
-        const int c = b % Cc;
-        const int rest = b / Cc;
-        const int q = rest % Mq;
-        const int base = (rest / Mq) * M;
-        const int j = q % L;
-        if (L > 1) {
+        const int c = b & cmask;
+        const int rest = b >> lgC;
+        const int q = rest & qmask;
+        const int base = (rest >> lgMq) << lgM;
+        const int j = q & lmask;
+        if (lgL > 0) {
             const C* t = tw + d.off[s] + j * (RP - 1);
 #pragma unroll
             for (int i = 1; i < RP; ++i) x[k * RP + i] = mul_w<INV>(x[k * RP + i], t[i - 1]);
         }
         dft_r<RP, INV>(x + k * RP);
-        const int ob = base + (q / L) * RP * L + j;
+        const int ob = base + ((q >> lgL) << (LGR + lgL)) + j;
 #pragma unroll
-        for (int m = 0; m < RP; ++m) lds[(ob + m * L) * Cc + c] = x[k * RP + m];
+        for (int m = 0; m < RP; ++m) lds[lds_ix(((ob + (m << lgL)) << lgC) + c)] = x[k * RP + m];
     }
     __syncthreads();
 }
 
-template <typename R, bool INV>
+// Tile element e (linear) -> (c, row, hb) in the given order.
+__device__ __forceinline__ void tile_coords(int e, int lgC, int lgM, int lgHb, int rowFast, int& c, int& row,
+                                            int& hb) {
+    c = e & ((1 << lgC) - 1);
+    const int rest = e >> lgC;
+    if (rowFast) {
+        row = rest & ((1 << lgM) - 1);
+        hb = rest >> lgM;
+    } else {
+        hb = rest & ((1 << lgHb) - 1);
+        row = rest >> lgHb;
+    }
+}
+
+template <typename R, bool INV, int E>
 __global__ void __launch_bounds__(1024)
-k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, DiagProg pre,
+k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
            const typename CT<R>::type* __restrict__ tw, const double2* __restrict__ tw4) {
     using C = typename CT<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int M = d.M, Cc = d.C, Hb = d.Hb;
-    const int tile = M * Cc * Hb;
+    const int lgM = d.lgM, lgC = d.lgC, lgHb = d.lgHb;
+    const int tile = 1 << (lgM + lgC + lgHb);
 
     uint64_t b = blockIdx.x;
-    const uint64_t nLoBlk = d.nLo / Cc, nH2Blk = d.nH2 / Hb;
-    const uint64_t loBlk = b % nLoBlk;
-    b /= nLoBlk;
-    const uint64_t h2Blk = b % nH2Blk;
-    const uint64_t h1 = b / nH2Blk;
-    const uint64_t lo0 = loBlk * Cc;
-    const uint64_t inBase = h1 * d.inH1 + h2Blk * Hb * d.inH2 + lo0;
-    const uint64_t outBase = h1 * d.outH1 + h2Blk * Hb * d.outH2 + lo0;
+    const uint64_t loBlk = b & ((1ull << d.lgNLoBlk) - 1);
+    b >>= d.lgNLoBlk;
+    const uint64_t h2Blk = b & ((1ull << d.lgNH2Blk) - 1);
+    const uint64_t h1 = b >> d.lgNH2Blk;
+    const uint64_t lo0 = loBlk << lgC;
+    const uint64_t inBase = h1 * d.inH1 + ((h2Blk << lgHb) * d.inH2) + lo0;
+    const uint64_t outBase = h1 * d.outH1 + ((h2Blk << lgHb) * d.outH2) + lo0;
 
-    for (int e = tid; e < tile; e += nt) {
-        const int c = e % Cc;
-        const int rest = e / Cc;
-        int row, hb;
-        if (d.rowFastIn) { row = rest % M; hb = rest / M; } else { hb = rest % Hb; row = rest / Hb; }
-        const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
-        C v = in[g];
-        if (d.usePre) v = apply_diag(pre, g, v);
-        lds[(hb * M + row) * Cc + c] = v;
+    // ---- load (pending phases applied per element)
+    if (sizeof(C) == 8 && d.vecIn) {
+        for (int e = 2 * tid; e < tile; e += 2 * nt) {
+            int c, row, hb, c1, row1, hb1;
+            tile_coords(e, lgC, lgM, lgHb, d.rowFastIn, c, row, hb);
+            tile_coords(e + 1, lgC, lgM, lgHb, d.rowFastIn, c1, row1, hb1);
+            const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
+            const float4 w = *reinterpret_cast<const float4*>(in + g);
+            C v0 = cmk<C>(w.x, w.y), v1 = cmk<C>(w.z, w.w);
+            if (d.usePre) {
+                v0 = apply_fast(pre, g, v0);
+                v1 = apply_fast(pre, g + 1, v1);
+            }
+            lds[lds_ix((((hb << lgM) + row) << lgC) + c)] = v0;
+            lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)] = v1;
+        }
+    } else {
+        for (int e = tid; e < tile; e += nt) {
+            int c, row, hb;
+            tile_coords(e, lgC, lgM, lgHb, d.rowFastIn, c, row, hb);
+            const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
+            C v = in[g];
+            if (d.usePre) v = apply_fast(pre, g, v);
+            lds[lds_ix((((hb << lgM) + row) << lgC) + c)] = v;
+        }
     }
     __syncthreads();
 
     for (int s = 0; s < d.nst; ++s) {
         const int rp = d.rad[s];
-        if (rp == 8) stockham_stage<8, INV>(lds, d, s, tw, tid, nt);
-        else if (rp == 4) stockham_stage<4, INV>(lds, d, s, tw, tid, nt);
-        else stockham_stage<2, INV>(lds, d, s, tw, tid, nt);
+        if (rp == 8) stockham_stage<8, E, INV>(lds, d, s, tw, tid, nt);
+        else if (rp == 4) stockham_stage<4, E, INV>(lds, d, s, tw, tid, nt);
+        else stockham_stage<2, E, INV>(lds, d, s, tw, tid, nt);
     }
 
+    // ---- store (four-step twiddle in FP64, 1/sqrt(N) as hi + lo)
     const R scale = (R)d.outScale;
     const R scaleLo = (R)(d.outScale - (double)scale);
     const uint64_t nmask = (1ull << d.tw4LogN) - 1ull;
-    for (int e = tid; e < tile; e += nt) {
-        const int c = e % Cc;
-        const int rest = e / Cc;
-        int row, hb;
-        if (d.rowFastOut) { row = rest % M; hb = rest / M; } else { hb = rest % Hb; row = rest / Hb; }
-        C v = lds[(hb * M + row) * Cc + c];
+    auto finish = [&](C v, int c, int row) -> C {
         if (d.tw4) {
             const uint64_t j1 = (lo0 + c) >> d.tw4Shift;
             const double2 w = mul_w<INV>(make_double2((double)v.x, (double)v.y),
@@ -351,14 +400,32 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         }
         v.x = fma(v.x, scaleLo, v.x * scale);
         v.y = fma(v.y, scaleLo, v.y * scale);
-        out[outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c] = v;
+        return v;
+    };
+    if (sizeof(C) == 8 && d.vecOut) {
+        for (int e = 2 * tid; e < tile; e += 2 * nt) {
+            int c, row, hb, c1, row1, hb1;
+            tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
+            tile_coords(e + 1, lgC, lgM, lgHb, d.rowFastOut, c1, row1, hb1);
+            const C v0 = finish(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], c, row);
+            const C v1 = finish(lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)], c1, row1);
+            const uint64_t g = outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c;
+            *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
+        }
+    } else {
+        for (int e = tid; e < tile; e += nt) {
+            int c, row, hb;
+            tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
+            const C v = finish(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], c, row);
+            out[outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c] = v;
+        }
     }
 }
 
 // Direct DFT for very small states (fewer than 8 elements per tile).
 template <typename R, bool INV>
 __global__ void k_dft_naive(const typename CT<R>::type* in, typename CT<R>::type* out, uint64_t total,
-                            int sh, int nb, DiagProg pre, int usePre, double scale,
+                            int sh, int nb, FastDiag pre, int usePre, double scale,
                             const double2* __restrict__ twN) {
     using C = typename CT<R>::type;
     const uint64_t M = 1ull << nb, fmask = (M - 1) << sh;
@@ -370,7 +437,7 @@ __global__ void k_dft_naive(const typename CT<R>::type* in, typename CT<R>::type
         for (uint64_t j = 0; j < M; ++j) {
             const uint64_t gi = base | (j << sh);
             C v = in[gi];
-            if (usePre) v = apply_diag(pre, gi, v);
+            if (usePre) v = apply_fast(pre, gi, v);
             const double2 p = mul_w<INV>(make_double2((double)v.x, (double)v.y), twN[(j * k) & (M - 1)]);
             ax += p.x;
             ay += p.y;
@@ -386,78 +453,101 @@ struct PassGeom {
     uint64_t inH1, inH2, inR, outH1, outH2, outR;
 };
 
-static int maxTile(int prec) { return prec == CVDV_C128 ? 4096 : 8192; }
-static int colWidth(int prec) { return prec == CVDV_C128 ? 8 : 16; }
+static int ilog2(uint64_t v) { return 63 - __builtin_clzll(v); }
+static size_t lds_bytes(uint64_t tile, size_t esz) { return (tile + tile / 32) * esz; }
 
-static bool fit_pass(const PassGeom& g, int prec, FftPass& d) {
-    const uint64_t W = colWidth(prec);
+// Tile limits: a strided tile stays within 64 KiB of LDS so several workgroups
+// share a CU; a whole contiguous register may take one large workgroup.
+static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
+    const size_t esz = prec == CVDV_C128 ? 16 : 8;
+    const uint64_t W = prec == CVDV_C128 ? 8 : 16;
     const uint64_t C = g.nLo < W ? g.nLo : W;
     const uint64_t tileCM = C * (uint64_t)g.M;
-    if (tileCM > (uint64_t)maxTile(prec)) return false;
+    const size_t small = 64 * 1024;
+    const bool big = allowBig && C == 1 && lds_bytes(tileCM, esz) <= device_max_lds() && tileCM <= 16384;
+    if (lds_bytes(tileCM, esz) > small && !big) return false;
     uint64_t Hb = 1;
-    const uint64_t target = 2048;
-    while (Hb * 2 <= g.nH2 && tileCM * Hb * 2 <= target) Hb *= 2;
+    while (Hb * 2 <= g.nH2 && tileCM * Hb * 2 <= 2048) Hb *= 2;
     if (tileCM * Hb < 8) return false;
-    d.M = g.M;
-    d.C = (int)C;
-    d.Hb = (int)Hb;
-    d.nLo = g.nLo;
-    d.nH2 = g.nH2;
-    d.nH1 = g.nH1;
+    const uint64_t tile = tileCM * Hb;
+    d.lgM = ilog2(g.M);
+    d.lgC = ilog2(C);
+    d.lgHb = ilog2(Hb);
+    d.lgNLoBlk = ilog2(g.nLo / C);
+    d.lgNH2Blk = ilog2(g.nH2 / Hb);
     d.inH1 = g.inH1; d.inH2 = g.inH2; d.inR = g.inR;
     d.outH1 = g.outH1; d.outH2 = g.outH2; d.outR = g.outR;
     d.rowFastIn = g.inR <= g.inH2 ? 1 : 0;
     d.rowFastOut = g.outR <= g.outH2 ? 1 : 0;
+    // 16-byte pairs along the fastest dimension when it is contiguous in memory
+    auto vec_ok = [&](int rowFast, uint64_t sR, uint64_t sH) {
+        if (esz != 8 || tile < 2) return 0;
+        if (C >= 2) return 1;
+        if (rowFast) return (sR == 1 && g.M >= 2) ? 1 : 0;
+        return (sH == 1 && Hb >= 2) ? 1 : 0;
+    };
+    d.vecIn = vec_ok(d.rowFastIn, g.inR, g.inH2);
+    d.vecOut = vec_ok(d.rowFastOut, g.outR, g.outH2);
     return true;
 }
 
 template <typename R>
 static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
-                        FftPass d, const DiagProg& pre, bool inverse, int prec) {
+                        FftPass d, const FastDiag& pre, bool inverse, int prec, uint64_t nH1) {
     using C = typename CT<R>::type;
-    const StagePlan sp = stage_plan(__builtin_ctz((unsigned)d.M));
+    const StagePlan sp = stage_plan(d.lgM);
     d.nst = sp.nst;
     for (int s = 0; s < sp.nst; ++s) {
         d.rad[s] = sp.rad[s];
-        d.L[s] = sp.L[s];
+        d.lgL[s] = ilog2((uint64_t)sp.L[s]);
         d.off[s] = sp.off[s];
     }
-    const C* tw = (const C*)stage_table(ctx, __builtin_ctz((unsigned)d.M), prec);
+    const C* tw = (const C*)stage_table(ctx, d.lgM, prec);
     const double2* tw4 = d.tw4 ? full_table(ctx, d.tw4LogN) : nullptr;
-    const int tile = d.M * d.C * d.Hb;
-    const int nt = tile / 8;
-    const uint64_t blocks = d.nH1 * (d.nH2 / d.Hb) * (d.nLo / d.C);
-    const size_t lds = (size_t)tile * sizeof(C);
+    const int tile = 1 << (d.lgM + d.lgC + d.lgHb);
+    const int E = tile / 8 > 1024 ? 16 : 8;
+    const int nt = tile / E;
+    const uint64_t blocks = nH1 << (d.lgNH2Blk + d.lgNLoBlk);
+    const size_t lds = lds_bytes(tile, sizeof(C));
     const int pid = prof_begin(ctx, KK_FFT);
-    if (inverse)
-        hipLaunchKernelGGL((k_fft_tile<R, true>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
-                           in, out, d, pre, tw, tw4);
-    else
-        hipLaunchKernelGGL((k_fft_tile<R, false>), dim3((unsigned)blocks), dim3(nt), lds, ctx->stream,
-                           in, out, d, pre, tw, tw4);
+#define CVDV_FFT_LAUNCH(INVV, EE)                                                                     \
+    do {                                                                                             \
+        auto kfn = k_fft_tile<R, INVV, EE>;                                                          \
+        if (lds > 64 * 1024)                                                                         \
+            HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                       (int)lds));                                                   \
+        hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(nt), lds, ctx->stream, in, out, d, pre, tw, tw4); \
+    } while (0)
+    if (inverse) {
+        if (E == 16) CVDV_FFT_LAUNCH(true, 16); else CVDV_FFT_LAUNCH(true, 8);
+    } else {
+        if (E == 16) CVDV_FFT_LAUNCH(false, 16); else CVDV_FFT_LAUNCH(false, 8);
+    }
+#undef CVDV_FFT_LAUNCH
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
 }
 
 // Runs the FFT over field [sh, sh+nb) of a buffer of `total` elements.
-// Returns true when the result landed in `tmp` (caller swaps buffers).
+// Returns true when the result landed in the buffer returned by tmpFn.
 template <typename R, typename TmpFn>
 static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, uint64_t total,
-                    int sh, int nb, bool inverse, const DiagProg& pre, double outScale, int prec) {
+                    int sh, int nb, bool inverse, const DiagProg& preProg, double outScale, int prec) {
     using C = typename CT<R>::type;
     const uint64_t N = 1ull << nb, S = 1ull << sh;
     const uint64_t H = total / (N * S);
-    const int usePre = (pre.n > 0 || pre.grev != 0.0) ? 1 : 0;
+    bool act;
+    const FastDiag pre = compile_diag(ctx, preProg, act);
+    const int usePre = act ? 1 : 0;
 
     // single pass: rows j (stride S), lo in [0,S), hi blocks in h2 (stride N*S)
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
-    g1.inH1 = 0; g1.outH1 = 0;
     FftPass d{};
-    if (N <= (uint64_t)maxTile(prec) && fit_pass(g1, prec, d)) {
+    if (fit_pass(g1, prec, d, true)) {
         d.tw4 = 0; d.tw4Shift = 0; d.tw4LogN = 0;
         d.usePre = usePre;
         d.outScale = outScale;
-        launch_pass<R>(ctx, data, data, d, pre, inverse, prec);
+        launch_pass<R>(ctx, data, data, d, pre, inverse, prec, 1);
         return false;
     }
     if (total < 4096 || N <= 64) {
@@ -485,7 +575,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     // pass 2: FFT over j1 (stride S), h2 = k2 (in stride N1*S, out stride S), out rows stride N2*S
     PassGeom p2{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, S, N2 * S};
     FftPass d2{};
-    if (!fit_pass(p1, prec, d1) || !fit_pass(p2, prec, d2))
+    if (!fit_pass(p1, prec, d1, false) || !fit_pass(p2, prec, d2, false))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
     d1.tw4 = 1; d1.tw4Shift = sh; d1.tw4LogN = nb;
     d1.usePre = usePre;
@@ -494,8 +584,8 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     d2.usePre = 0;
     d2.outScale = outScale;
     C* tmp = tmpFn();
-    launch_pass<R>(ctx, data, data, d1, pre, inverse, prec);
-    launch_pass<R>(ctx, data, tmp, d2, DiagProg{}, inverse, prec);
+    launch_pass<R>(ctx, data, data, d1, pre, inverse, prec, H);
+    launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, inverse, prec, H);
     return true;
 }
 

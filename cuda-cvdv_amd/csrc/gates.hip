@@ -12,6 +12,7 @@
 #include <string>
 
 #include "cvdv_internal.h"
+#include "diag_fast.h"
 #include "../../include/cvdv_hip.h"
 
 static const double PI = 3.14159265358979323846;
@@ -19,10 +20,44 @@ static const double SQRT2 = 1.41421356237309504880;
 
 // ---------------------------------------------------------------- kernels
 template <typename C>
-__global__ void k_diag(C* __restrict__ s, uint64_t total, DiagProg p) {
+__global__ void k_diag(C* __restrict__ s, uint64_t total, FastDiag f) {
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x)
-        s[g] = apply_diag(p, g, s[g]);
+        s[g] = apply_fast(f, g, s[g]);
+}
+
+struct TabMake {
+    int kind, nb;
+    uint32_t off;
+    int pad;
+    double c1, c2, c3;
+};
+
+struct TabMakeArgs {
+    int n;
+    uint32_t total;
+    TabMake t[CVDV_MAX_TERMS];
+};
+
+// e^{2 pi i phase(j)} for every tabulated term, FP64 angle and sin/cos,
+// rounded once to the state's precision.
+template <typename C>
+__global__ void k_make_tables(C* __restrict__ arena, TabMakeArgs a) {
+    for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.total; e += gridDim.x * blockDim.x) {
+        int i = 0;
+        while (i + 1 < a.n && e >= a.t[i + 1].off) ++i;
+        const TabMake& t = a.t[i];
+        const uint64_t j = e - t.off;
+        double v;
+        if (t.kind == DT_POLY) {
+            const double m = (double)((int64_t)(2 * j) + 1 - ((int64_t)1 << t.nb));
+            v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
+        } else {
+            v = j ? t.c1 : -t.c1;
+        }
+        const double2 u = unit_rev(cvdv_frac(v), 0.0);
+        arena[e] = cmk<C>(u.x, u.y);
+    }
 }
 
 // 2x2 unitary on the pair (bit = 0, bit = 1); pending phases applied on load.
@@ -30,7 +65,7 @@ __global__ void k_diag(C* __restrict__ s, uint64_t total, DiagProg p) {
 // rounded once per gate and never sees a rounded (non-unitary) matrix.
 template <typename C>
 __global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, double2 m00, double2 m01,
-                        double2 m10, double2 m11, DiagProg p, int usep) {
+                        double2 m10, double2 m11, FastDiag p, int usep) {
     const uint64_t lowmask = (1ull << bit) - 1ull;
     for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
          pid += (uint64_t)gridDim.x * blockDim.x) {
@@ -38,8 +73,8 @@ __global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, double2 m00
         const uint64_t i1 = i0 | (1ull << bit);
         C a = s[i0], b = s[i1];
         if (usep) {
-            a = apply_diag(p, i0, a);
-            b = apply_diag(p, i1, b);
+            a = apply_fast(p, i0, a);
+            b = apply_fast(p, i1, b);
         }
         const double2 ad = make_double2((double)a.x, (double)a.y);
         const double2 bd = make_double2((double)b.x, (double)b.y);
@@ -54,7 +89,7 @@ __global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, double2 m00
 // src/cuda/kernels/gates.cuh:134-157 and torchCvdv.py:122-137).
 template <typename C>
 __global__ void k_cparity(C* __restrict__ s, uint64_t npairs, int sh, int nb, int ctrlPos,
-                          int ctrlFlip, int tgtRefl, DiagProg p, int usep) {
+                          int ctrlFlip, int tgtRefl, FastDiag p, int usep) {
     const int top = sh + nb - 1;
     const uint64_t lowmask = (1ull << top) - 1ull;
     const uint64_t fmask = ((1ull << nb) - 1ull) << sh;
@@ -64,8 +99,8 @@ __global__ void k_cparity(C* __restrict__ s, uint64_t npairs, int sh, int nb, in
         const uint64_t i1 = i0 ^ fmask;
         C a = s[i0], b = s[i1];
         if (usep) {
-            a = apply_diag(p, i0, a);
-            b = apply_diag(p, i1, b);
+            a = apply_fast(p, i0, a);
+            b = apply_fast(p, i1, b);
         }
         const uint64_t lower = tgtRefl ? i1 : i0;
         const int cb = (int)((lower >> ctrlPos) & 1ull) ^ ctrlFlip;
@@ -94,8 +129,56 @@ DiagProg ctx_take_pending(CVDVContext* ctx) {
     return p;
 }
 
-void launch_diag(CVDVContext* ctx, const DiagProg& p) {
+FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active) {
+    FastDiag f{};
+    active = prog_active(p);
+    const double ga = 2.0 * PI * p.grev;
+    f.gphase = make_double2(std::cos(ga), std::sin(ga));
+    if (!active) return f;
+    TabMakeArgs a{};
+    uint32_t off = 0;
+    for (int i = 0; i < p.n; ++i) {
+        const DiagTerm& t = p.t[i];
+        const bool tab = (t.kind == DT_POLY && t.nb1 <= 20) || t.kind == DT_ZBIT;
+        if (!tab) {
+            f.gen.t[f.gen.n++] = t;
+            continue;
+        }
+        const int nb = t.kind == DT_ZBIT ? 1 : t.nb1;
+        f.tab[f.ntab++] = DiagTab{t.sh1, nb, t.ctrl, off};
+        a.t[a.n++] = TabMake{t.kind, nb, off, 0, t.c1, t.c2, t.c3};
+        off += 1u << nb;
+    }
+    f.usegen = f.gen.n > 0;
+    if (a.n) {
+        a.total = off;
+        const size_t need = (size_t)off * ctx->elemBytes();
+        if (need > ctx->arenaBytes) {
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            if (ctx->arena) HIPCHK(hipFree(ctx->arena));
+            ctx->arena = nullptr;
+            ctx->arenaBytes = 0;
+            const size_t sz = need < (size_t)(1 << 20) ? (size_t)(1 << 20) : need;
+            HIPCHK(hipMalloc(&ctx->arena, sz));
+            ctx->arenaBytes = sz;
+        }
+        const unsigned blocks = (unsigned)((off + 255) / 256);
+        if (ctx->prec == CVDV_C128)
+            hipLaunchKernelGGL(k_make_tables<double2>, dim3(blocks), dim3(256), 0, ctx->stream,
+                               (double2*)ctx->arena, a);
+        else
+            hipLaunchKernelGGL(k_make_tables<float2>, dim3(blocks), dim3(256), 0, ctx->stream,
+                               (float2*)ctx->arena, a);
+        HIPCHK(hipGetLastError());
+    }
+    f.arena = ctx->arena;
+    return f;
+}
+
+void launch_diag(CVDVContext* ctx, const DiagProg& pp) {
     const uint64_t n = ctx->totalSize;
+    bool act;
+    const FastDiag p = compile_diag(ctx, pp, act);
     const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_diag<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
@@ -233,8 +316,9 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
         for (int i = 0; i < 8; ++i) mm[i] = t[i];
     }
     const int bit = (int)ctx_ctrl_bit(ctx, reg, qubit);
-    DiagProg p = ctx_take_pending(ctx);
-    const int usep = prog_active(p) ? 1 : 0;
+    bool act;
+    const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
+    const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const double2 m00 = make_double2(mm[0], mm[1]), m01 = make_double2(mm[2], mm[3]);
     const double2 m10 = make_double2(mm[4], mm[5]), m11 = make_double2(mm[6], mm[7]);
@@ -287,8 +371,9 @@ static void parity(CVDVContext* ctx, int reg) { ctx->refl[reg] ^= 1; }
 static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
     const int nb = ctx->nq[tgt];
     if (nb == 0) return;
-    DiagProg p = ctx_take_pending(ctx);
-    const int usep = prog_active(p) ? 1 : 0;
+    bool act;
+    const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
+    const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const int cpos = (int)ctx_ctrl_bit(ctx, creg, cq);
     const int pid = prof_begin(ctx, KK_CPARITY);

@@ -11,7 +11,7 @@
 //     or 8 (c128) columns for a strided register, or a whole contiguous
 //     register of up to 2^14 (c64) / 2^13 (c128) points in one workgroup;
 //   * otherwise the four-step split N = N1 * N2 (j = j1 + N1 j2):
-//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2} (FP64);
+//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2};
 //       pass 2 (out of place, transposing): FFT over j1, written at k2 + N2 k1.
 // Pending diagonal phases are applied (compiled, see diag_fast.h) as elements
 // are loaded by the first pass and 1/sqrt(N) as they are stored by the last.
@@ -36,7 +36,7 @@ struct TwKey {
 
 struct TwiddleCache {
     std::map<TwKey, void*> stage;  // per-stage Stockham tables for length M
-    std::map<TwKey, void*> full;   // W_N^e, e in [0, N), FP64
+    std::map<TwKey, void*> full;   // W_N^e, e in [0, N)
 };
 
 void twiddle_cache_free(TwiddleCache* tc) {
@@ -161,18 +161,18 @@ static void* stage_table(CVDVContext* ctx, int logM, int prec) {
     return d;
 }
 
-// W_N^e for e in [0, N), always FP64 (the four-step twiddle and the direct DFT
-// apply it in FP64 for both precisions).
-static const double2* full_table(CVDVContext* ctx, int logN) {
+// W_N^e for e in [0, N) in the given precision (FP32: error-diffused rounding,
+// as for the stage tables).  The direct DFT always uses the FP64 table.
+static const void* full_table(CVDVContext* ctx, int logN, int prec = CVDV_C128) {
     TwiddleCache* tc = cache(ctx);
-    auto it = tc->full.find({logN, CVDV_C128});
-    if (it != tc->full.end()) return (const double2*)it->second;
+    auto it = tc->full.find({logN, prec});
+    if (it != tc->full.end()) return it->second;
     const uint64_t N = 1ull << logN;
     std::vector<double> re(N), im(N);
     for (uint64_t e = 0; e < N; ++e) host_twiddle(e, N, re[e], im[e]);
-    void* d = upload_table(re, im, CVDV_C128);
-    tc->full[{logN, CVDV_C128}] = d;
-    return (const double2*)d;
+    void* d = upload_table(re, im, prec);
+    tc->full[{logN, prec}] = d;
+    return d;
 }
 
 size_t device_max_lds() {
@@ -192,12 +192,13 @@ struct FftPass {
     int lgNLoBlk, lgNH2Blk;        // grid = nH1 * nH2Blk * nLoBlk
     int rowFastIn, rowFastOut;     // linear order after the column index
     int vecIn, vecOut;             // 1: pairs of complex64 move as one 16-byte access
+    int directOut;                 // 1: last stage stores straight to global (coalesced rows)
     int nst;
     int rad[6], lgL[6], off[6];
     int tw4, tw4Shift, tw4LogN;
     int usePre;
-    uint64_t inH1, inH2, inR;
-    uint64_t outH1, outH2, outR;
+    int lgInH1, lgInH2, lgInR;     // every stride is a power of two
+    int lgOutH1, lgOutH2, lgOutR;
     double outScale;  // applied as hi + lo parts (FMA) so an FP32 store keeps it unbiased
 };
 
@@ -276,48 +277,6 @@ __device__ __forceinline__ void dft_r(C* x) {
     else dft8<INV>(x);
 }
 
-// One Stockham stage: each thread owns E/RP butterflies of radix RP.
-template <int RP, int E, bool INV, typename C>
-__device__ __forceinline__ void stockham_stage(C* lds, const FftPass& d, int s, const C* __restrict__ tw,
-                                               int tid, int nt) {
-    constexpr int NB = E / RP;
-    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgM = d.lgM, lgC = d.lgC, lgL = d.lgL[s];
-    const int lgMq = lgM - LGR;
-    const int cmask = (1 << lgC) - 1, qmask = (1 << lgMq) - 1, lmask = (1 << lgL) - 1;
-    C x[E];
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int b = tid + k * nt;
-        const int c = b & cmask;
-        const int rest = b >> lgC;
-        const int q = rest & qmask;
-        const int base = (rest >> lgMq) << lgM;
-#pragma unroll
-        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_ix(((base + q + (i << lgMq)) << lgC) + c)];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const int b = tid + k * nt;
-        const int c = b & cmask;
-        const int rest = b >> lgC;
-        const int q = rest & qmask;
-        const int base = (rest >> lgMq) << lgM;
-        const int j = q & lmask;
-        if (lgL > 0) {
-            const C* t = tw + d.off[s] + j * (RP - 1);
-#pragma unroll
-            for (int i = 1; i < RP; ++i) x[k * RP + i] = mul_w<INV>(x[k * RP + i], t[i - 1]);
-        }
-        dft_r<RP, INV>(x + k * RP);
-        const int ob = base + ((q >> lgL) << (LGR + lgL)) + j;
-#pragma unroll
-        for (int m = 0; m < RP; ++m) lds[lds_ix(((ob + (m << lgL)) << lgC) + c)] = x[k * RP + m];
-    }
-    __syncthreads();
-}
-
 // Tile element e (linear) -> (c, row, hb) in the given order.
 __device__ __forceinline__ void tile_coords(int e, int lgC, int lgM, int lgHb, int rowFast, int& c, int& row,
                                             int& hb) {
@@ -332,10 +291,139 @@ __device__ __forceinline__ void tile_coords(int e, int lgC, int lgM, int lgHb, i
     }
 }
 
+// Stockham stage geometry.  Butterfly b of a stage of radix RP (sub-length
+// L = 2^lgL already transformed) takes rows q + i*M/RP of column c in block hb
+// and writes rows ob + m*L, ob = hb*M + (q/L)*RP*L + (q mod L); every thread
+// owns E/RP butterflies b = tid + k*nt.
+struct BflyPos {
+    int c, q, hb;
+};
+
+template <int RP>
+__device__ __forceinline__ BflyPos bfly_pos(int b, int lgC, int lgM) {
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgMq = lgM - LGR;
+    BflyPos p;
+    p.c = b & ((1 << lgC) - 1);
+    const int rest = b >> lgC;
+    p.q = rest & ((1 << lgMq) - 1);
+    p.hb = rest >> lgMq;
+    return p;
+}
+
+// Twiddle the inputs of the butterflies in x (stage s, radix RP) and transform.
+template <int RP, int E, bool INV, typename C>
+__device__ __forceinline__ void stage_compute(C* x, const FftPass& d, int s, const C* __restrict__ tw,
+                                              int tid, int nt) {
+    constexpr int NB = E / RP;
+    const int lgL = d.lgL[s];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        if (lgL > 0) {
+            const BflyPos p = bfly_pos<RP>(tid + k * nt, d.lgC, d.lgM);
+            const C* t = tw + d.off[s] + (p.q & ((1 << lgL) - 1)) * (RP - 1);
+#pragma unroll
+            for (int i = 1; i < RP; ++i) x[k * RP + i] = mul_w<INV>(x[k * RP + i], t[i - 1]);
+        }
+        dft_r<RP, INV>(x + k * RP);
+    }
+}
+
+// Outputs of stage s (radix RP) -> LDS in Stockham order.
+template <int RP, int E, typename C>
+__device__ __forceinline__ void stage_write(C* lds, const C* x, const FftPass& d, int s, int tid, int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgC = d.lgC, lgM = d.lgM, lgL = d.lgL[s];
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const int ob = (p.hb << lgM) + ((p.q >> lgL) << (LGR + lgL)) + (p.q & ((1 << lgL) - 1));
+#pragma unroll
+        for (int m = 0; m < RP; ++m) lds[lds_ix(((ob + (m << lgL)) << lgC) + p.c)] = x[k * RP + m];
+    }
+}
+
+// Inputs of stage s (radix RP) <- LDS.
+template <int RP, int E, typename C>
+__device__ __forceinline__ void stage_read(const C* lds, C* x, const FftPass& d, int tid, int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const int base = (p.hb << lgM) + p.q;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_ix(((base + (i << lgMq)) << lgC) + p.c)];
+    }
+}
+
+// Stage-0 inputs straight from global memory, pending phases applied.
+template <int RP, int E, typename C>
+__device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
+                                            const FastDiag& pre, int tid, int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
+#pragma unroll
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = in[g0 + ((uint64_t)(i << lgMq) << d.lgInR)];
+        if (d.usePre) {
+#pragma unroll
+            for (int i = 0; i < RP; ++i)
+                x[k * RP + i] = apply_fast(pre, g0 + ((uint64_t)(i << lgMq) << d.lgInR), x[k * RP + i]);
+        }
+    }
+}
+
+template <bool INV, typename R, typename C>
+__device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int row, const C* __restrict__ tw4,
+                                        R scale, R scaleLo) {
+    if (d.tw4) {
+        const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
+        v = mul_w<INV>(v, tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)]);
+    }
+    v.x = fma(v.x, scaleLo, v.x * scale);
+    v.y = fma(v.y, scaleLo, v.y * scale);
+    return v;
+}
+
+// Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
+template <int RP, int E, bool INV, typename R, typename C>
+__device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
+                                           uint64_t lo0, const C* __restrict__ tw4, R scale, R scaleLo, int tid,
+                                           int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const uint64_t g0 = outBase + ((uint64_t)p.hb << d.lgOutH2) + p.c;
+#pragma unroll
+        for (int m = 0; m < RP; ++m) {
+            const int row = p.q + (m << lgMq);
+            out[g0 + ((uint64_t)row << d.lgOutR)] =
+                out_finish<INV>(x[k * RP + m], d, lo0 + p.c, row, tw4, scale, scaleLo);
+        }
+    }
+}
+
+#define CVDV_RADIX_SWITCH(RPV, CALL)                    \
+    do {                                                \
+        if ((RPV) == 8) { constexpr int RP = 8; CALL; } \
+        else if ((RPV) == 4) { constexpr int RP = 4; CALL; } \
+        else { constexpr int RP = 2; CALL; }            \
+    } while (0)
+
 template <typename R, bool INV, int E>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
-           const typename CT<R>::type* __restrict__ tw, const double2* __restrict__ tw4) {
+           const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
     using C = typename CT<R>::type;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     C* lds = reinterpret_cast<C*>(smem_raw);
@@ -349,75 +437,50 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     const uint64_t h2Blk = b & ((1ull << d.lgNH2Blk) - 1);
     const uint64_t h1 = b >> d.lgNH2Blk;
     const uint64_t lo0 = loBlk << lgC;
-    const uint64_t inBase = h1 * d.inH1 + ((h2Blk << lgHb) * d.inH2) + lo0;
-    const uint64_t outBase = h1 * d.outH1 + ((h2Blk << lgHb) * d.outH2) + lo0;
-
-    // ---- load (pending phases applied per element)
-    if (sizeof(C) == 8 && d.vecIn) {
-        for (int e = 2 * tid; e < tile; e += 2 * nt) {
-            int c, row, hb, c1, row1, hb1;
-            tile_coords(e, lgC, lgM, lgHb, d.rowFastIn, c, row, hb);
-            tile_coords(e + 1, lgC, lgM, lgHb, d.rowFastIn, c1, row1, hb1);
-            const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
-            const float4 w = *reinterpret_cast<const float4*>(in + g);
-            C v0 = cmk<C>(w.x, w.y), v1 = cmk<C>(w.z, w.w);
-            if (d.usePre) {
-                v0 = apply_fast(pre, g, v0);
-                v1 = apply_fast(pre, g + 1, v1);
-            }
-            lds[lds_ix((((hb << lgM) + row) << lgC) + c)] = v0;
-            lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)] = v1;
-        }
-    } else {
-        for (int e = tid; e < tile; e += nt) {
-            int c, row, hb;
-            tile_coords(e, lgC, lgM, lgHb, d.rowFastIn, c, row, hb);
-            const uint64_t g = inBase + (uint64_t)hb * d.inH2 + (uint64_t)row * d.inR + c;
-            C v = in[g];
-            if (d.usePre) v = apply_fast(pre, g, v);
-            lds[lds_ix((((hb << lgM) + row) << lgC) + c)] = v;
-        }
-    }
-    __syncthreads();
-
-    for (int s = 0; s < d.nst; ++s) {
-        const int rp = d.rad[s];
-        if (rp == 8) stockham_stage<8, E, INV>(lds, d, s, tw, tid, nt);
-        else if (rp == 4) stockham_stage<4, E, INV>(lds, d, s, tw, tid, nt);
-        else stockham_stage<2, E, INV>(lds, d, s, tw, tid, nt);
-    }
-
-    // ---- store (four-step twiddle in FP64, 1/sqrt(N) as hi + lo)
+    const uint64_t inBase = (h1 << d.lgInH1) + ((h2Blk << lgHb) << d.lgInH2) + lo0;
+    const uint64_t outBase = (h1 << d.lgOutH1) + ((h2Blk << lgHb) << d.lgOutH2) + lo0;
     const R scale = (R)d.outScale;
     const R scaleLo = (R)(d.outScale - (double)scale);
-    const uint64_t nmask = (1ull << d.tw4LogN) - 1ull;
-    auto finish = [&](C v, int c, int row) -> C {
-        if (d.tw4) {
-            const uint64_t j1 = (lo0 + c) >> d.tw4Shift;
-            const double2 w = mul_w<INV>(make_double2((double)v.x, (double)v.y),
-                                         tw4[(j1 * (uint64_t)row) & nmask]);
-            v = cmk<C>(w.x, w.y);
-        }
-        v.x = fma(v.x, scaleLo, v.x * scale);
-        v.y = fma(v.y, scaleLo, v.y * scale);
-        return v;
-    };
+
+    C x[E];
+    // stage 0: loads of the first butterflies, phases, first transform (no twiddles)
+    CVDV_RADIX_SWITCH(d.rad[0], (stage0_load<RP, E>(in, x, d, inBase, pre, tid, nt),
+                                 stage_compute<RP, E, INV>(x, d, 0, tw, tid, nt)));
+    for (int s = 1; s < d.nst; ++s) {
+        CVDV_RADIX_SWITCH(d.rad[s - 1], (stage_write<RP, E>(lds, x, d, s - 1, tid, nt)));
+        __syncthreads();
+        CVDV_RADIX_SWITCH(d.rad[s], (stage_read<RP, E>(lds, x, d, tid, nt)));
+        __syncthreads();
+        CVDV_RADIX_SWITCH(d.rad[s], (stage_compute<RP, E, INV>(x, d, s, tw, tid, nt)));
+    }
+    const int last = d.nst - 1;
+    if (d.directOut) {
+        CVDV_RADIX_SWITCH(d.rad[last], (last_store<RP, E, INV>(out, x, d, outBase, lo0, tw4, scale, scaleLo,
+                                                               tid, nt)));
+        return;
+    }
+    // staged store: natural order through LDS, written in the output's contiguous order
+    CVDV_RADIX_SWITCH(d.rad[last], (stage_write<RP, E>(lds, x, d, last, tid, nt)));
+    __syncthreads();
     if (sizeof(C) == 8 && d.vecOut) {
         for (int e = 2 * tid; e < tile; e += 2 * nt) {
             int c, row, hb, c1, row1, hb1;
             tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
             tile_coords(e + 1, lgC, lgM, lgHb, d.rowFastOut, c1, row1, hb1);
-            const C v0 = finish(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], c, row);
-            const C v1 = finish(lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)], c1, row1);
-            const uint64_t g = outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c;
+            const C v0 = out_finish<INV>(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], d, lo0 + c, row, tw4,
+                                         scale, scaleLo);
+            const C v1 = out_finish<INV>(lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)], d, lo0 + c1, row1,
+                                         tw4, scale, scaleLo);
+            const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
             *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
         }
     } else {
         for (int e = tid; e < tile; e += nt) {
             int c, row, hb;
             tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
-            const C v = finish(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], c, row);
-            out[outBase + (uint64_t)hb * d.outH2 + (uint64_t)row * d.outR + c] = v;
+            const C v = out_finish<INV>(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], d, lo0 + c, row, tw4,
+                                        scale, scaleLo);
+            out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = v;
         }
     }
 }
@@ -475,8 +538,9 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     d.lgHb = ilog2(Hb);
     d.lgNLoBlk = ilog2(g.nLo / C);
     d.lgNH2Blk = ilog2(g.nH2 / Hb);
-    d.inH1 = g.inH1; d.inH2 = g.inH2; d.inR = g.inR;
-    d.outH1 = g.outH1; d.outH2 = g.outH2; d.outR = g.outR;
+    auto lg0 = [](uint64_t v) { return v ? ilog2(v) : 0; };  // stride 0: that index is always 0
+    d.lgInH1 = lg0(g.inH1); d.lgInH2 = lg0(g.inH2); d.lgInR = lg0(g.inR);
+    d.lgOutH1 = lg0(g.outH1); d.lgOutH2 = lg0(g.outH2); d.lgOutR = lg0(g.outR);
     d.rowFastIn = g.inR <= g.inH2 ? 1 : 0;
     d.rowFastOut = g.outR <= g.outH2 ? 1 : 0;
     // 16-byte pairs along the fastest dimension when it is contiguous in memory
@@ -488,6 +552,9 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     };
     d.vecIn = vec_ok(d.rowFastIn, g.inR, g.inH2);
     d.vecOut = vec_ok(d.rowFastOut, g.outR, g.outH2);
+    // last stage writes rows q + m*M/R of column c: coalesced when a full
+    // column group is one 128-byte run or when rows follow the columns directly
+    d.directOut = (C == W || g.outR == C) ? 1 : 0;
     return true;
 }
 
@@ -503,7 +570,7 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
         d.off[s] = sp.off[s];
     }
     const C* tw = (const C*)stage_table(ctx, d.lgM, prec);
-    const double2* tw4 = d.tw4 ? full_table(ctx, d.tw4LogN) : nullptr;
+    const C* tw4 = d.tw4 ? (const C*)full_table(ctx, d.tw4LogN, prec) : nullptr;
     const int tile = 1 << (d.lgM + d.lgC + d.lgHb);
     const int E = tile / 8 > 1024 ? 16 : 8;
     const int nt = tile / E;
@@ -552,7 +619,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     }
     if (total < 4096 || N <= 64) {
         // tiny state: direct DFT, out of place
-        const double2* twN = full_table(ctx, nb);
+        const double2* twN = (const double2*)full_table(ctx, nb);
         C* tmp = tmpFn();
         uint64_t blocks = (total + 255) / 256;
         const int pid = prof_begin(ctx, KK_FFT);

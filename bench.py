@@ -14,7 +14,13 @@ diagonal phases are fused and count zero.  value = passes/s in the mixed mode
 the mixed/FP64 speedup and the fidelity loss of mixed against FP64 are
 reported beside it.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+The same JSON line carries the batched-circuit workload under "c4": 4096
+independent circuits ([1-3 qubits, one oscillator of 2^8-2^12 points], 50-200
+gates from {h, rx, rz, cd, r, s, cp}, seed 4) run by one launch, against the
+same circuits issued one by one (per-gate kernel launches), and the mixed vs
+complex128 agreement of their qubit marginals.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline] [--no-c4]
 """
 
 from __future__ import annotations
@@ -89,6 +95,162 @@ def squeezed_vacuum(nq, r):
     dx = math.sqrt(2 * math.pi / N)
     x = (np.arange(N) - (N - 1) / 2) * dx
     return (math.exp(r / 2) * math.pi ** -0.25 * math.sqrt(dx) * np.exp(-0.5 * x * x * math.exp(2 * r))).astype(np.complex128)
+
+
+# ---------------------------------------------------------------- batched circuits
+C4_N = 4096
+C4_SEED = 4
+
+
+def coherent(nq, alpha):
+    """pi^-1/4 sqrt(dx) e^{-(x-q)^2/2} e^{i(p x - p q/2)}, q = sqrt2 Re a, p = sqrt2 Im a."""
+    N = 1 << nq
+    dx = math.sqrt(2 * math.pi / N)
+    x = (np.arange(N) - (N - 1) / 2) * dx
+    q, p = math.sqrt(2) * alpha.real, math.sqrt(2) * alpha.imag
+    return math.pi ** -0.25 * math.sqrt(dx) * np.exp(-0.5 * (x - q) ** 2) * np.exp(1j * (p * x - p * q / 2))
+
+
+def make_c4(n=C4_N, seed=C4_SEED):
+    """Circuits (shape, init, gates): shape [q, nb] with q ~ U{1,2,3} qubits as register 0 and
+    an oscillator of 2^nb points, nb ~ U{8..12}, as register 1; the qubits start in |0..0>,
+    the oscillator in a coherent state |alpha| <= 2 (uniform disc) or squeezed vacuum
+    r ~ U(0, 0.8) (50/50); 50-200 gates uniform over {h, rx, rz, cd, r, s, cp} with
+    angles ~ U(-pi, pi), cd |alpha| <= 0.5 (uniform disc), squeeze r ~ U(-0.3, 0.3)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        q = int(rng.integers(1, 4))
+        nb = int(rng.integers(8, 13))
+        if rng.random() < 0.5:
+            rad, ph = 2 * math.sqrt(rng.random()), rng.uniform(0, 2 * math.pi)
+            init = ("coherent", complex(rad * math.cos(ph), rad * math.sin(ph)))
+        else:
+            init = ("squeezed", float(rng.uniform(0, 0.8)))
+        gates = []
+        for _ in range(int(rng.integers(50, 201))):
+            k = int(rng.integers(0, 7))
+            qb = int(rng.integers(0, q))
+            if k == 0:
+                gates.append(("h", 0, qb))
+            elif k == 1:
+                gates.append(("rx", 0, qb, float(rng.uniform(-math.pi, math.pi))))
+            elif k == 2:
+                gates.append(("rz", 0, qb, float(rng.uniform(-math.pi, math.pi))))
+            elif k == 3:
+                rad, ph = 0.5 * math.sqrt(rng.random()), rng.uniform(0, 2 * math.pi)
+                gates.append(("cd", 1, 0, qb, complex(rad * math.cos(ph), rad * math.sin(ph))))
+            elif k == 4:
+                gates.append(("r", 1, float(rng.uniform(-math.pi, math.pi))))
+            elif k == 5:
+                gates.append(("s", 1, float(rng.uniform(-0.3, 0.3))))
+            else:
+                gates.append(("cp", 1, 0, qb))
+        out.append(([q, nb], init, gates))
+    return out
+
+
+def c4_init(shape, init):
+    qa = np.zeros(1 << shape[0], dtype=np.complex128)
+    qa[0] = 1.0
+    cv = coherent(shape[1], init[1]) if init[0] == "coherent" else squeezed_vacuum(shape[1], init[1])
+    return [qa, cv.astype(np.complex128)]
+
+
+def c4_ft_count(g) -> int:
+    if g[0] == "cd":
+        return 2 if abs(g[4].real) > 1e-12 else 0
+    if g[0] in ("r", "s"):
+        return ft_count(g)
+    return 0
+
+
+def run_c4(prec, circuits, n_seq, steps):
+    import torch
+    from hipcvdv import HipCvdv, HipCvdvBatch
+
+    n = len(circuits)
+    batch = HipCvdvBatch([c[0] for c in circuits], precision=prec)
+    t0 = time.perf_counter()
+    for i, (_, _, gates) in enumerate(circuits):
+        sim = batch.circuit(i)
+        for g in gates:
+            getattr(sim, g[0])(*g[1:])
+    record_s = time.perf_counter() - t0
+    batch.initFromSeparable([c4_init(c[0], c[1]) for c in circuits])
+    tune_us = batch.autotune(reps=2)
+    threads, lds = batch.getLaunch()
+
+    def batched():
+        batch.run()
+        return batch.measure(0), batch.norms()
+
+    batched()
+    batch.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        probs, norms = batched()
+    el = (time.perf_counter() - t0) / steps
+    # the same circuits issued one by one: one context per shape, reused
+    sims = {}
+    seps = []
+    for shape, init, _ in circuits[:n_seq]:
+        if tuple(shape) not in sims:
+            sims[tuple(shape)] = HipCvdv(shape, precision=prec)
+        seps.append([torch.from_numpy(a).to("cuda") for a in c4_init(shape, init)])
+
+    def seq_one(i):
+        shape, _, gates = circuits[i]
+        sim = sims[tuple(shape)]
+        ptrs = [a.data_ptr() for a in seps[i]]
+        import ctypes
+        arr = (ctypes.c_void_p * 2)(*ptrs)
+        sim._c("cvdvInitFromSeparable", arr, ctypes.c_int(2))
+        for g in gates:
+            getattr(sim, g[0])(*g[1:])
+        return sim.m(0), sim.m()
+
+    seq_one(0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    seq_out = [seq_one(i) for i in range(n_seq)]
+    seq_el = time.perf_counter() - t0
+    agree = max(float(np.max(np.abs(seq_out[i][0] - probs[i]))) for i in range(n_seq))
+    passes = sum(c4_ft_count(g) for c in circuits for g in c[2])
+    amps = sum(1 << (c[0][0] + c[0][1]) for c in circuits)
+    del sims, seps
+    return {"batched_ms": el * 1e3, "circuits_per_s": n / el, "record_s": record_s,
+            "launch": {"threads": threads, "lds_bytes": lds, "autotuned_us": tune_us},
+            "num_ops": batch.numOps(), "fft_passes": passes, "amplitudes": amps,
+            "seq_circuits": n_seq, "seq_ms_per_circuit": seq_el / n_seq * 1e3,
+            "seq_circuits_per_s": n_seq / seq_el,
+            "speedup_vs_sequential": (seq_el / n_seq * n) / el,
+            "seq_vs_batched_max_prob_diff": agree,
+            "probs0": probs, "norms": norms}
+
+
+def c4_section(n, n_seq, steps):
+    circuits = make_c4(n)
+    res = {}
+    for prec in ("mixed", "complex128"):
+        res[prec] = run_c4(prec, circuits, n_seq, steps)
+        print(f"[bench] c4 {prec}: {res[prec]['batched_ms']:.2f} ms for {n} circuits, "
+              f"{res[prec]['circuits_per_s']:.0f} circuits/s; sequential "
+              f"{res[prec]['seq_ms_per_circuit']:.2f} ms/circuit; speedup "
+              f"{res[prec]['speedup_vs_sequential']:.1f}x", file=sys.stderr, flush=True)
+    pm, pd = res["mixed"].pop("probs0"), res["complex128"].pop("probs0")
+    nm, nd = res["mixed"].pop("norms"), res["complex128"].pop("norms")
+    m = res["mixed"]
+    return {
+        "workload": f"{n} circuits [1-3 qubits, 2^8-2^12 oscillator], 50-200 gates "
+                    "{h, rx, rz, cd, r, s, cp}, seed 4",
+        "metric": "circuits/s, one batched launch vs sequential per-gate launches",
+        "value": m["circuits_per_s"], "unit": "circuits/s", "precision": "mixed",
+        "mixed": m, "complex128": res["complex128"],
+        "mixed_vs_fp64_max_prob_diff": max(float(np.max(np.abs(a - b))) for a, b in zip(pm, pd)),
+        "max_norm_dev_mixed": float(np.max(np.abs(nm - 1.0))),
+        "max_norm_dev_fp64": float(np.max(np.abs(nd - 1.0))),
+    }
 
 
 def run_gpu(prec, ops, steps, warmup):
@@ -192,12 +354,19 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--only", choices=["mixed", "complex128"], default=None,
                     help="time a single precision (profiling runs)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched-circuit workload")
+    ap.add_argument("--c4-circuits", type=int, default=C4_N)
+    ap.add_argument("--c4-seq", type=int, default=64, help="circuits timed one by one")
+    ap.add_argument("--c4-only", action="store_true", help="run only the batched-circuit workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; the state is not sharded across GPUs "
-              "(see DESIGN.md, 'Sharding').", file=sys.stderr)
+        print("bench.py: runs on one GPU only; the state vector is not sharded across GPUs.",
+              file=sys.stderr)
         sys.exit(2)
 
+    if args.c4_only:
+        print(json.dumps(c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))))
+        return
     ops = make_circuit()
     passes = sum(ft_count(op) for op in ops)
     A = 1 << sum(REGS)
@@ -256,6 +425,8 @@ def main():
         out["max_rel_amp_err_mixed_vs_fp64"] = rel_err
         out["norm_fp64"] = n0
         out["norm_mixed"] = n1
+    if not args.no_c4 and not args.only:
+        out["c4"] = c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ops, args.cpu_seconds)
     print(json.dumps(out))

@@ -248,7 +248,7 @@ void ctx_materialize(CVDVContext* ctx) {
 }
 
 // ---------------------------------------------------------------- C ABI
-static CVDVContext* create_impl(int numReg, int* numQubits, int precision) {
+static CVDVContext* create_impl(int numReg, const int* numQubits, int precision, bool recording = false) {
     CVDVContext* ctx = new CVDVContext{};
     if (numReg <= 0 || numQubits == nullptr) return ctx;  // context.cu:11-13
     if (numReg > CVDV_MAX_REG) {
@@ -283,7 +283,21 @@ static CVDVContext* create_impl(int numReg, int* numQubits, int precision) {
         ctx->flw[i] = f;  // context.cu:33-38
     }
     reset_layout(ctx);
+    if (recording) {
+        ctx->rec = new BatchRecording();
+        ctx->initialized = true;  // gates record from the canonical layout
+        return ctx;
+    }
     HIPCHK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    return ctx;
+}
+
+CVDVContext* ctx_create_recording(int numReg, const int* numQubits, int precision) {
+    CVDVContext* ctx = create_impl(numReg, numQubits, precision, true);
+    if (!ctx->rec) {  // invalid shape: the error is recorded
+        delete ctx;
+        return nullptr;
+    }
     return ctx;
 }
 
@@ -304,14 +318,14 @@ CVDVContext* cvdvCreateEx(int numReg, int* numQubits, int precision) {
 int cvdvGetPrecision(CVDVContext* ctx) { return ctx ? ctx->prec : -1; }
 
 void cvdvDestroy(CVDVContext* ctx) {
-    if (!ctx) return;
+    if (!ctx || refuse_recording(ctx)) return;
     ctx_release(ctx);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
 
 void cvdvFree(CVDVContext* ctx) {
-    if (!ctx) return;
+    if (!ctx || refuse_recording(ctx)) return;
     ctx_release(ctx);  // context.cu:120-129: drop state and metadata, keep the handle
     ctx->numReg = 0;
     ctx->totalQbt = 0;
@@ -336,7 +350,7 @@ void cvdvSynchronize(CVDVContext* ctx) {
 }
 
 void cvdvInitFromSeparable(CVDVContext* ctx, void** devicePtrs, int numReg) {
-    if (!ctx) return;
+    if (!ctx || refuse_recording(ctx)) return;
     if (ctx->numReg == 0) {
         cvdv_set_error(CVDV_ERR_STATE,
                        "Error: Must call cvdvCreate with registers before cvdvInitFromSeparable");
@@ -382,7 +396,7 @@ void cvdvInitFromSeparable(CVDVContext* ctx, void** devicePtrs, int numReg) {
 }
 
 void cvdvSetStateFromDevicePtr(CVDVContext* ctx, void* d_src) {
-    if (!ctx || !d_src) return;
+    if (!ctx || !d_src || refuse_recording(ctx)) return;
     if (ctx->numReg == 0) {
         cvdv_set_error(CVDV_ERR_STATE, "Error: context has no registers");
         return;
@@ -465,14 +479,14 @@ void cvdvProfileReset(CVDVContext* ctx) {
 }
 
 void cvdvFlush(CVDVContext* ctx) {
-    if (!ctx || !ctx->initialized) return;
+    if (!ctx || !ctx->initialized || refuse_recording(ctx)) return;
     CVDV_TRY
     ctx_materialize(ctx);
     CVDV_CATCH()
 }
 
 void cvdvGetState(CVDVContext* ctx, double* realOut, double* imagOut) {
-    if (!ctx) return;
+    if (!ctx || refuse_recording(ctx)) return;
     if (!ctx->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "Error: State not initialized. Call cvdvInitStateVector first.");
         return;
@@ -495,7 +509,7 @@ void cvdvGetState(CVDVContext* ctx, double* realOut, double* imagOut) {
 }
 
 void cvdvGetStateDevice(CVDVContext* ctx, void* d_dst) {
-    if (!ctx || !d_dst) return;
+    if (!ctx || !d_dst || refuse_recording(ctx)) return;
     if (!ctx->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "Error: State not initialized");
         return;

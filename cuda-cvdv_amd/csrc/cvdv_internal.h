@@ -67,25 +67,32 @@ __device__ __forceinline__ double diag_field_m(uint64_t g, int sh, int nb) {
     return (double)((int64_t)(2 * j) + 1 - ((int64_t)1 << nb));
 }
 
+// One term's phase of element g in revolutions (not yet reduced).
+__device__ __forceinline__ double diag_term_rev(const DiagTerm& t, uint64_t g) {
+    double v;
+    if (t.kind == DT_POLY) {
+        const double m = diag_field_m(g, t.sh1, t.nb1);
+        v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
+    } else if (t.kind == DT_BILIN) {
+        const double m1 = diag_field_m(g, t.sh1, t.nb1);
+        const double m2 = diag_field_m(g, t.sh2, t.nb2);
+        v = t.c1 * (m1 * m2);  // m1*m2 is an exact integer below 2^53
+    } else {
+        v = ((g >> t.sh1) & 1ull) ? t.c1 : -t.c1;
+    }
+    if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) v = -v;
+    return v;
+}
+
 // Phase of element g in revolutions, reduced to [-1/2, 1/2] term by term.
+// The loop is unrolled to constant term indices so a program passed as a
+// kernel argument is read in place (a runtime index would make the compiler
+// copy the whole argument into per-lane scratch memory).
 __device__ __forceinline__ double diag_rev(const DiagProg& p, uint64_t g) {
     double r = p.grev;
-    for (int i = 0; i < p.n; ++i) {
-        const DiagTerm& t = p.t[i];
-        double v;
-        if (t.kind == DT_POLY) {
-            const double m = diag_field_m(g, t.sh1, t.nb1);
-            v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
-        } else if (t.kind == DT_BILIN) {
-            const double m1 = diag_field_m(g, t.sh1, t.nb1);
-            const double m2 = diag_field_m(g, t.sh2, t.nb2);
-            v = t.c1 * (m1 * m2);  // m1*m2 is an exact integer below 2^53
-        } else {
-            v = ((g >> t.sh1) & 1ull) ? t.c1 : -t.c1;
-        }
-        if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) v = -v;
-        r += cvdv_frac(v);
-    }
+#pragma unroll
+    for (int i = 0; i < CVDV_MAX_TERMS; ++i)
+        if (i < p.n) r += cvdv_frac(diag_term_rev(p.t[i], g));
     return cvdv_frac(r);
 }
 
@@ -133,6 +140,26 @@ __device__ __forceinline__ C apply_diag(const DiagProg& p, uint64_t g, C v) {
     return cmul(v, unit_rev(diag_rev(p, g), R(0)));
 }
 
+// ---------------------------------------------------------------- batch recording
+// A recording context (one per circuit of a batch, batch.hip) runs the same gate
+// compiler but appends these ops instead of launching kernels.
+enum BatchOpKind { BOP_FFT = 1, BOP_QUBIT = 2, BOP_CPARITY = 3, BOP_DIAG = 4 };
+
+struct BatchOp {
+    int kind;
+    int sh, nb, inv;     // register field (FFT, CPARITY target); inv: inverse FFT
+    int bit;             // QUBIT: qubit bit; CPARITY: control bit
+    int ctrlFlip, tgtRefl;
+    int prog;            // index of the phase program applied on load, -1 = none
+    double outScale;     // FFT: 1/sqrt(N)
+    double2 m[4];        // QUBIT: 2x2 matrix, FP64
+};
+
+struct BatchRecording {
+    std::vector<BatchOp> ops;
+    std::vector<DiagProg> progs;
+};
+
 // ---------------------------------------------------------------- context
 struct TwiddleCache;  // fft.hip
 
@@ -165,6 +192,7 @@ struct CVDVContext {
     size_t scratchBytes = 0;
     void* arena = nullptr;            // phase tables of the compiled pending program
     size_t arenaBytes = 0;
+    BatchRecording* rec = nullptr;    // non-null: record ops, launch nothing
     bool profOn = false;
     std::vector<ProfRec> prof;        // recorded launches
     std::vector<hipEvent_t> evPool;   // reusable events
@@ -209,6 +237,17 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
 
 // scratch
 void* ctx_scratch(CVDVContext* ctx, size_t bytes);
+
+// batch recording (gates.hip): append an op whose phase program is `p`
+void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p);
+// readouts refuse recording contexts (no state of their own)
+bool refuse_recording(CVDVContext* ctx);
+// Stockham stage twiddles of length 2^logM in the given precision (fft.hip):
+// device table (cached in tc) and a host copy for tables built elsewhere.
+const void* fft_stage_table(TwiddleCache*& tc, int logM, int prec);
+std::vector<uint8_t> fft_stage_host(int logM, int prec);
+// A context for one circuit of a batch: records gates, owns no state (ctx.hip).
+CVDVContext* ctx_create_recording(int numReg, const int* numQubits, int precision);
 
 // profiling (ctx.hip): bracket a launch on ctx->stream when profiling is on
 int prof_begin(CVDVContext* ctx, int kind);

@@ -31,13 +31,15 @@ struct FastDiag {
 
 // Mixed precision: the angle is already reduced to [-1/2, 1/2] revolutions in
 // FP64, so the hardware sin/cos (which take revolutions) apply directly.  Their
-// error is ~2 ulp but |e|^2 - 1 averages -6.5e-8; one FP32 renormalisation step
-// removes that bias (measured: hardware 1.25e-7 max abs error).
+// error is ~2 ulp but |e|^2 - 1 averages -6.5e-8, which would shrink the norm
+// on every phase pass.  One Newton step k = (3 - |e|^2)/2 removes the bias; it
+// runs in FP64 because in FP32 a factor within 6e-8 of 1 rounds to exactly 1.
+// The final rounding of c k and s k to FP32 is unbiased.
 __device__ __forceinline__ float2 unit_rev_fast(double f, float) {
     const float ff = (float)f;
-    const float s = __builtin_amdgcn_sinf(ff), c = __builtin_amdgcn_cosf(ff);
-    const float k = fmaf(-0.5f, fmaf(c, c, s * s), 1.5f);
-    return make_float2(c * k, s * k);
+    const double s = (double)__builtin_amdgcn_sinf(ff), c = (double)__builtin_amdgcn_cosf(ff);
+    const double k = fma(-0.5, fma(c, c, s * s), 1.5);
+    return make_float2((float)(c * k), (float)(s * k));
 }
 __device__ __forceinline__ double2 unit_rev_fast(double f, double) { return unit_rev(f, 0.0); }
 
@@ -46,11 +48,14 @@ __device__ __forceinline__ C apply_fast(const FastDiag& f, uint64_t g, C v) {
     using R = decltype(v.x);
     C fac = cmk<C>(f.gphase.x, f.gphase.y);
     const C* ar = (const C*)f.arena;
-    for (int i = 0; i < f.ntab; ++i) {
-        const DiagTab& t = f.tab[i];
-        C e = ar[t.off + ((g >> t.sh) & ((1ull << t.nb) - 1ull))];
-        if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) e.y = -e.y;
-        fac = cmul(fac, e);
+#pragma unroll
+    for (int i = 0; i < CVDV_MAX_TERMS; ++i) {  // constant indices: see diag_rev
+        if (i < f.ntab) {
+            const DiagTab& t = f.tab[i];
+            C e = ar[t.off + ((g >> t.sh) & ((1ull << t.nb) - 1ull))];
+            if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) e.y = -e.y;
+            fac = cmul(fac, e);
+        }
     }
     if (f.usegen) fac = cmul(fac, unit_rev_fast(diag_rev(f.gen, g), R(0)));
     return cmul(v, fac);

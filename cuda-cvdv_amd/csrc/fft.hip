@@ -24,6 +24,7 @@
 
 #include "cvdv_internal.h"
 #include "diag_fast.h"
+#include "fft_bfly.h"
 
 static const long double PI_L = 3.141592653589793238462643383279502884L;
 
@@ -109,27 +110,35 @@ static void round_unit_diffused(const double* re, const double* im, size_t n, fl
     }
 }
 
-static void* upload_table(const std::vector<double>& re, const std::vector<double>& im, int prec,
-                          const std::vector<size_t>& segments = {}) {
+// Host copy of a table in the state's precision (FP32: error-diffused per segment).
+static std::vector<uint8_t> round_table(const std::vector<double>& re, const std::vector<double>& im, int prec,
+                                        const std::vector<size_t>& segments = {}) {
     const size_t n = re.size();
-    void* d = nullptr;
+    std::vector<uint8_t> bytes;
     if (prec == CVDV_C128) {
-        std::vector<double2> h(n ? n : 1);
+        bytes.resize(n * sizeof(double2));
+        double2* h = (double2*)bytes.data();
         for (size_t i = 0; i < n; ++i) h[i] = make_double2(re[i], im[i]);
-        HIPCHK(hipMalloc(&d, h.size() * sizeof(double2)));
-        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(double2), hipMemcpyHostToDevice));
     } else {
-        std::vector<float2> h(n ? n : 1);
+        bytes.resize(n * sizeof(float2));
+        float2* h = (float2*)bytes.data();
         size_t start = 0;
         std::vector<size_t> seg = segments;
         seg.push_back(n);
         for (size_t end : seg) {
-            if (end > start) round_unit_diffused(re.data() + start, im.data() + start, end - start, h.data() + start);
+            if (end > start) round_unit_diffused(re.data() + start, im.data() + start, end - start, h + start);
             start = end;
         }
-        HIPCHK(hipMalloc(&d, h.size() * sizeof(float2)));
-        HIPCHK(hipMemcpy(d, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice));
     }
+    return bytes;
+}
+
+static void* upload_table(const std::vector<double>& re, const std::vector<double>& im, int prec,
+                          const std::vector<size_t>& segments = {}) {
+    const std::vector<uint8_t> h = round_table(re, im, prec, segments);
+    void* d = nullptr;
+    HIPCHK(hipMalloc(&d, h.size() ? h.size() : 16));
+    if (!h.empty()) HIPCHK(hipMemcpy(d, h.data(), h.size(), hipMemcpyHostToDevice));
     return d;
 }
 
@@ -138,13 +147,13 @@ static TwiddleCache* cache(CVDVContext* ctx) {
     return ctx->tw;
 }
 
-static void* stage_table(CVDVContext* ctx, int logM, int prec) {
-    TwiddleCache* tc = cache(ctx);
-    auto it = tc->stage.find({logM, prec});
-    if (it != tc->stage.end()) return it->second;
+// Stockham stage twiddles for length 2^logM: stage s (radix R, sub-length L)
+// holds W_{RL}^{ij} at off[s] + j (R-1) + i - 1, j < L, 1 <= i < R.
+static void stage_values(int logM, std::vector<double>& re, std::vector<double>& im, std::vector<size_t>& segs) {
     const StagePlan p = stage_plan(logM);
     const uint64_t M = 1ull << logM;
-    std::vector<double> re(p.total), im(p.total);
+    re.assign(p.total, 0.0);
+    im.assign(p.total, 0.0);
     for (int s = 0; s < p.nst; ++s) {
         const int R = p.rad[s], L = p.L[s];
         for (int j = 0; j < L; ++j)
@@ -154,11 +163,32 @@ static void* stage_table(CVDVContext* ctx, int logM, int prec) {
                 host_twiddle(e, M, re[p.off[s] + j * (R - 1) + i - 1], im[p.off[s] + j * (R - 1) + i - 1]);
             }
     }
-    std::vector<size_t> segs;
+    segs.clear();
     for (int st = 1; st < p.nst; ++st) segs.push_back((size_t)p.off[st]);
+}
+
+std::vector<uint8_t> fft_stage_host(int logM, int prec) {
+    std::vector<double> re, im;
+    std::vector<size_t> segs;
+    stage_values(logM, re, im, segs);
+    return round_table(re, im, prec, segs);
+}
+
+const void* fft_stage_table(TwiddleCache*& tcp, int logM, int prec) {
+    if (!tcp) tcp = new TwiddleCache();
+    TwiddleCache* tc = tcp;
+    auto it = tc->stage.find({logM, prec});
+    if (it != tc->stage.end()) return it->second;
+    std::vector<double> re, im;
+    std::vector<size_t> segs;
+    stage_values(logM, re, im, segs);
     void* d = upload_table(re, im, prec, segs);
     tc->stage[{logM, prec}] = d;
     return d;
+}
+
+static void* stage_table(CVDVContext* ctx, int logM, int prec) {
+    return (void*)fft_stage_table(ctx->tw, logM, prec);
 }
 
 // W_N^e for e in [0, N) in the given precision (FP32: error-diffused rounding,
@@ -205,77 +235,7 @@ struct FftPass {
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
-// ---------------------------------------------------------------- butterflies
-template <bool INV, typename C>
-__device__ __forceinline__ C mul_w(C a, C w) {
-    return INV ? cmulc(a, w) : cmul(a, w);
-}
-
-template <bool INV, typename C>
-__device__ __forceinline__ C rot_mi(C a) {  // a * (-i) forward, a * (+i) inverse
-    C r;
-    if (INV) { r.x = -a.y; r.y = a.x; } else { r.x = a.y; r.y = -a.x; }
-    return r;
-}
-
-template <typename C> __device__ __forceinline__ C cadd(C a, C b) { C r; r.x = a.x + b.x; r.y = a.y + b.y; return r; }
-template <typename C> __device__ __forceinline__ C csub(C a, C b) { C r; r.x = a.x - b.x; r.y = a.y - b.y; return r; }
-
-template <bool INV, typename C>
-__device__ __forceinline__ void dft2(C* x) {
-    const C a = x[0], b = x[1];
-    x[0] = cadd(a, b);
-    x[1] = csub(a, b);
-}
-
-template <bool INV, typename C>
-__device__ __forceinline__ void dft4(C& x0, C& x1, C& x2, C& x3) {
-    const C t0 = cadd(x0, x2), t1 = csub(x0, x2);
-    const C t2 = cadd(x1, x3), t3 = rot_mi<INV>(csub(x1, x3));
-    x0 = cadd(t0, t2);
-    x2 = csub(t0, t2);
-    x1 = cadd(t1, t3);
-    x3 = csub(t1, t3);
-}
-
-// v / sqrt(2).  In FP32 the constant is carried as hi + lo and applied with an
-// FMA: a single rounded constant is 1.1e-8 low, which would shrink the norm by
-// ~3e-8 on every radix-8 stage instead of leaving an unbiased rounding error.
-__device__ __forceinline__ double mul_rsqrt2(double v) { return v * 0.70710678118654752440; }
-__device__ __forceinline__ float mul_rsqrt2(float v) {
-    const float hi = 0.707106769084930419921875f;  // nearest float below 1/sqrt(2)
-    const float lo = 1.2101617485882343e-08f;      // 1/sqrt(2) - hi
-    return fmaf(v, lo, v * hi);
-}
-
-template <bool INV, typename C>
-__device__ __forceinline__ void dft8(C* x) {
-    C a0 = x[0], a1 = x[2], a2 = x[4], a3 = x[6];
-    C b0 = x[1], b1 = x[3], b2 = x[5], b3 = x[7];
-    dft4<INV>(a0, a1, a2, a3);
-    dft4<INV>(b0, b1, b2, b3);
-    // W8^1 = (1 - i)/sqrt2, W8^2 = -i, W8^3 = (-1 - i)/sqrt2 (forward)
-    C w1b, w3b;
-    if (!INV) {
-        w1b.x = mul_rsqrt2(b1.x + b1.y); w1b.y = mul_rsqrt2(b1.y - b1.x);
-        w3b.x = mul_rsqrt2(b3.y - b3.x); w3b.y = -mul_rsqrt2(b3.x + b3.y);
-    } else {
-        w1b.x = mul_rsqrt2(b1.x - b1.y); w1b.y = mul_rsqrt2(b1.y + b1.x);
-        w3b.x = -mul_rsqrt2(b3.x + b3.y); w3b.y = mul_rsqrt2(b3.x - b3.y);
-    }
-    const C w2b = rot_mi<INV>(b2);
-    x[0] = cadd(a0, b0); x[4] = csub(a0, b0);
-    x[1] = cadd(a1, w1b); x[5] = csub(a1, w1b);
-    x[2] = cadd(a2, w2b); x[6] = csub(a2, w2b);
-    x[3] = cadd(a3, w3b); x[7] = csub(a3, w3b);
-}
-
-template <int RP, bool INV, typename C>
-__device__ __forceinline__ void dft_r(C* x) {
-    if (RP == 2) dft2<INV>(x);
-    else if (RP == 4) dft4<INV>(x[0], x[1], x[2], x[3]);
-    else dft8<INV>(x);
-}
+// ---------------------------------------------------------------- butterflies (fft_bfly.h)
 
 // Tile element e (linear) -> (c, row, hb) in the given order.
 __device__ __forceinline__ void tile_coords(int e, int lgC, int lgM, int lgHb, int rowFast, int& c, int& row,
@@ -371,11 +331,9 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
         const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
         const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
 #pragma unroll
-        for (int i = 0; i < RP; ++i) x[k * RP + i] = in[g0 + ((uint64_t)(i << lgMq) << d.lgInR)];
-        if (d.usePre) {
-#pragma unroll
-            for (int i = 0; i < RP; ++i)
-                x[k * RP + i] = apply_fast(pre, g0 + ((uint64_t)(i << lgMq) << d.lgInR), x[k * RP + i]);
+        for (int i = 0; i < RP; ++i) {
+            const uint64_t g = g0 + ((uint64_t)(i << lgMq) << d.lgInR);
+            x[k * RP + i] = d.usePre ? apply_fast(pre, g, in[g]) : in[g];
         }
     }
 }
@@ -387,8 +345,8 @@ __device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int
         const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
         v = mul_w<INV>(v, tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)]);
     }
-    v.x = fma(v.x, scaleLo, v.x * scale);
-    v.y = fma(v.y, scaleLo, v.y * scale);
+    v.x = fma(v.x, scale, v.x * scaleLo);  // exact v*hi + rounded v*lo, one rounding
+    v.y = fma(v.y, scale, v.y * scaleLo);
     return v;
 }
 

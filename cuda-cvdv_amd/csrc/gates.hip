@@ -31,6 +31,7 @@ struct TabMake {
     uint32_t off;
     int pad;
     double c1, c2, c3;
+    double g;  // constant phase folded into this table (revolutions)
 };
 
 struct TabMakeArgs {
@@ -55,7 +56,7 @@ __global__ void k_make_tables(C* __restrict__ arena, TabMakeArgs a) {
         } else {
             v = j ? t.c1 : -t.c1;
         }
-        const double2 u = unit_rev(cvdv_frac(v), 0.0);
+        const double2 u = unit_rev(cvdv_frac(cvdv_frac(v) + t.g), 0.0);
         arena[e] = cmk<C>(u.x, u.y);
     }
 }
@@ -150,6 +151,20 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active) {
         off += 1u << nb;
     }
     f.usegen = f.gen.n > 0;
+    // The global phase is folded into an FP64-evaluated angle (an unconditional
+    // table, else the per-element program) instead of a constant factor rounded
+    // to FP32, whose |g|^2 - 1 would scale the norm by the same amount on every pass.
+    bool folded = false;
+    for (int i = 0; i < a.n && !folded; ++i)
+        if (f.tab[i].ctrl < 0) {
+            a.t[i].g = cvdv_frac(p.grev);
+            folded = true;
+        }
+    if (!folded && f.usegen) {
+        f.gen.grev = cvdv_frac(p.grev);
+        folded = true;
+    }
+    if (folded) f.gphase = make_double2(1.0, 0.0);
     if (a.n) {
         a.total = off;
         const size_t need = (size_t)off * ctx->elemBytes();
@@ -175,7 +190,30 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active) {
     return f;
 }
 
+void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p) {
+    op.prog = -1;
+    if (prog_active(p)) {
+        op.prog = (int)ctx->rec->progs.size();
+        ctx->rec->progs.push_back(p);
+    }
+    ctx->rec->ops.push_back(op);
+}
+
+bool refuse_recording(CVDVContext* ctx) {
+    if (ctx && ctx->rec) {
+        cvdv_set_error(CVDV_ERR_STATE, "batch circuit context: run the batch and read it through the batch API");
+        return true;
+    }
+    return false;
+}
+
 void launch_diag(CVDVContext* ctx, const DiagProg& pp) {
+    if (ctx->rec) {
+        BatchOp op{};
+        op.kind = BOP_DIAG;
+        rec_op(ctx, op, pp);
+        return;
+    }
     const uint64_t n = ctx->totalSize;
     bool act;
     const FastDiag p = compile_diag(ctx, pp, act);
@@ -300,7 +338,17 @@ void ctx_ft(CVDVContext* ctx, int reg, bool inverse) {
         }
     }
     prog_add(pre, tw);
-    fft_register(ctx, ctx->pshift[reg], nb, inverse, pre, 1.0 / std::sqrt((double)N));
+    if (ctx->rec) {
+        BatchOp op{};
+        op.kind = BOP_FFT;
+        op.sh = ctx->pshift[reg];
+        op.nb = nb;
+        op.inv = inverse ? 1 : 0;
+        op.outScale = 1.0 / std::sqrt((double)N);
+        rec_op(ctx, op, pre);
+    } else {
+        fft_register(ctx, ctx->pshift[reg], nb, inverse, pre, 1.0 / std::sqrt((double)N));
+    }
     ctx->pending = DiagProg{};
     prog_add(ctx->pending, tw);
     const double gphase = (double)(N - 1) * (double)(N - 1) / (4.0 * (double)N);
@@ -316,6 +364,14 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
         for (int i = 0; i < 8; ++i) mm[i] = t[i];
     }
     const int bit = (int)ctx_ctrl_bit(ctx, reg, qubit);
+    if (ctx->rec) {
+        BatchOp op{};
+        op.kind = BOP_QUBIT;
+        op.bit = bit;
+        for (int i = 0; i < 4; ++i) op.m[i] = make_double2(mm[2 * i], mm[2 * i + 1]);
+        rec_op(ctx, op, ctx_take_pending(ctx));
+        return;
+    }
     bool act;
     const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
     const int usep = act ? 1 : 0;
@@ -371,11 +427,22 @@ static void parity(CVDVContext* ctx, int reg) { ctx->refl[reg] ^= 1; }
 static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
     const int nb = ctx->nq[tgt];
     if (nb == 0) return;
+    const int cpos = (int)ctx_ctrl_bit(ctx, creg, cq);
+    if (ctx->rec) {
+        BatchOp op{};
+        op.kind = BOP_CPARITY;
+        op.sh = ctx->pshift[tgt];
+        op.nb = nb;
+        op.bit = cpos;
+        op.ctrlFlip = ctx->refl[creg];
+        op.tgtRefl = ctx->refl[tgt];
+        rec_op(ctx, op, ctx_take_pending(ctx));
+        return;
+    }
     bool act;
     const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
     const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
-    const int cpos = (int)ctx_ctrl_bit(ctx, creg, cq);
     const int pid = prof_begin(ctx, KK_CPARITY);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_cparity<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,

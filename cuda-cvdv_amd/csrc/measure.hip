@@ -300,7 +300,7 @@ static inline bool prog_on(const DiagProg& p) { return p.n > 0 || p.grev != 0.0;
 extern "C" {
 
 void cvdvMeasureMultiple(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut) {
-    if (!ctx || !regIdxs || numRegs <= 0 || !probsOut) return;
+    if (!ctx || !regIdxs || numRegs <= 0 || !probsOut || refuse_recording(ctx)) return;
     if (!ctx->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
         return;
@@ -326,7 +326,7 @@ void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, do
 
 double cvdvGetNorm(CVDVContext* ctx) {
     // measure.cu:276-300: sum |psi|^2 (not its square root)
-    if (!ctx) return 0.0;
+    if (!ctx || refuse_recording(ctx)) return 0.0;
     if (!ctx->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
         return 0.0;
@@ -340,7 +340,7 @@ double cvdvGetNorm(CVDVContext* ctx) {
 
 void cvdvGetFidelity(CVDVContext* ctx, void** devicePtrs, int numReg, double* fidOut) {
     // measure.cu:302-318: |<psi|sep>|^2
-    if (!ctx || !fidOut) return;
+    if (!ctx || !fidOut || refuse_recording(ctx)) return;
     if (!ctx->initialized || ctx->numReg == 0) {
         cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
         *fidOut = 0.0;
@@ -381,7 +381,7 @@ void cvdvGetFidelity(CVDVContext* ctx, void** devicePtrs, int numReg, double* fi
 
 double cvdvGetPhotonNumber(CVDVContext* ctx, int regIdx) {
     // measure.cu:320-331: (<x^2> + <p^2> - 1) / 2 through an ftQ2P / ftP2Q round trip
-    if (!ctx || !ctx->initialized) return -1.0;
+    if (!ctx || !ctx->initialized || refuse_recording(ctx)) return -1.0;
     if (regIdx < 0 || regIdx >= ctx->numReg) return -1.0;
     CVDV_TRY
     const int N = 1 << ctx->nq[regIdx];
@@ -408,7 +408,7 @@ void cvdvFidelityStatevectors(CVDVContext* ctx1, CVDVContext* ctx2, double* fidO
     // measure.cu:333-366: |<psi1|psi2>|^2
     if (!fidOut) return;
     *fidOut = 0.0;
-    if (!ctx1 || !ctx2) return;
+    if (!ctx1 || !ctx2 || refuse_recording(ctx1) || refuse_recording(ctx2)) return;
     if (!ctx1->initialized || !ctx2->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "cvdvFidelityStatevectors: state not initialized");
         return;

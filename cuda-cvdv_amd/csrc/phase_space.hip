@@ -315,7 +315,7 @@ static void get_husimi_wigner(CVDVContext* ctx, int r, double* out) {
     }
 
 static bool ps_ok(CVDVContext* ctx, int r, double* out) {
-    if (!ctx || !out) return false;
+    if (!ctx || !out || refuse_recording(ctx)) return false;
     if (!ctx->initialized) {
         cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
         return false;

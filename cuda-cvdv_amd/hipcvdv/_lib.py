@@ -68,6 +68,21 @@ SIGNATURES = {
     "cvdvProfileEnable": ([c_void_p, c_int], None),
     "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),
     "cvdvProfileReset": ([c_void_p], None),
+    "cvdvBatchCreate": ([c_int, POINTER(c_int), POINTER(c_int), c_int], c_void_p),
+    "cvdvBatchDestroy": ([c_void_p], None),
+    "cvdvBatchNumCircuits": ([c_void_p], c_int),
+    "cvdvBatchCircuit": ([c_void_p, c_int], c_void_p),
+    "cvdvBatchStateSize": ([c_void_p, c_int], c_size_t),
+    "cvdvBatchNumOps": ([c_void_p], c_size_t),
+    "cvdvBatchInitFromSeparable": ([c_void_p, POINTER(c_double)], None),
+    "cvdvBatchSetLaunch": ([c_void_p, c_int, c_int], None),
+    "cvdvBatchGetLaunch": ([c_void_p, POINTER(c_int), POINTER(c_int)], None),
+    "cvdvBatchAutotune": ([c_void_p, c_int], c_double),
+    "cvdvBatchRun": ([c_void_p], None),
+    "cvdvBatchSynchronize": ([c_void_p], None),
+    "cvdvBatchMeasure": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvBatchNorms": ([c_void_p, POINTER(c_double)], None),
+    "cvdvBatchGetState": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], None),
 }
 
 

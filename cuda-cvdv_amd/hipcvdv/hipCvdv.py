@@ -48,8 +48,25 @@ class HipCvdv:
         self.ctx = self._lib.cvdvCreateEx(len(numQubits_list), arr, PRECISIONS[precision])
         _lib.check()
 
+    @classmethod
+    def _recording(cls, lib, ctx: int, numQubits_list: List[int], precision: str, owner: Any) -> "HipCvdv":
+        """Wrap a batch circuit's recording context (gates only; owned by ``owner``)."""
+        self = cls.__new__(cls)
+        self.num_registers = len(numQubits_list)
+        self.register_dims = [1 << q for q in numQubits_list]
+        self.total_size = int(np.prod(self.register_dims, dtype=np.int64))
+        self.qubit_counts = np.array(numQubits_list, dtype=np.int32)
+        self.grid_steps = np.array([np.sqrt(2 * np.pi / d) for d in self.register_dims], dtype=np.float64)
+        self.precision = precision
+        self._lib = lib
+        self.ctx = ctx
+        self._owner = owner  # keeps the batch (and so the context) alive
+        return self
+
     def __del__(self):
         try:
+            if getattr(self, "_owner", None) is not None:
+                return
             if getattr(self, "ctx", None):
                 self._lib.cvdvDestroy(self.ctx)
                 self.ctx = None

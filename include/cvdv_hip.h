@@ -106,6 +106,42 @@ void   cvdvProfileEnable(CVDVContext* ctx, int on);
 int    cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs);
 void   cvdvProfileReset(CVDVContext* ctx);
 
+/* ---- batched circuits (addition) ----
+ * Many small independent circuits executed by one kernel launch, one
+ * workgroup per circuit.  numQubitsFlat lists every circuit's register sizes
+ * in order (numRegs[i] entries for circuit i); at most 8 registers of at most
+ * 12 qubits and 20 qubits in total per circuit.  cvdvBatchCircuit returns a
+ * recording context: every gate entry point above applies to it and records
+ * the gate (same decompositions); readouts and state setters on it fail with
+ * CVDV_ERR_STATE, and it is destroyed with the batch.  The initial state of
+ * each circuit is the product of its register arrays (hostC128: interleaved
+ * re/im doubles, circuits in order, registers in order).  cvdvBatchRun starts
+ * every circuit from that initial state and runs its recorded gates
+ * (asynchronous; recordings may be extended between runs).  Measure returns,
+ * for every circuit, the marginal of register regIdx (canonical order,
+ * 2^n entries per circuit, concatenated); Norms one sum of |psi|^2 per
+ * circuit; both are deterministic compensated FP64 sums. */
+typedef struct CVDVBatch CVDVBatch;
+CVDVBatch* cvdvBatchCreate(int numCircuits, const int* numRegs, const int* numQubitsFlat, int precision);
+void   cvdvBatchDestroy(CVDVBatch* batch);
+int    cvdvBatchNumCircuits(CVDVBatch* batch);
+CVDVContext* cvdvBatchCircuit(CVDVBatch* batch, int i);
+size_t cvdvBatchStateSize(CVDVBatch* batch, int i);
+size_t cvdvBatchNumOps(CVDVBatch* batch);
+void   cvdvBatchInitFromSeparable(CVDVBatch* batch, const double* hostC128);
+/* Launch shape: threads per workgroup (256 or 512) and the LDS budget in
+ * bytes; circuits whose state exceeds the budget run from a global work area.
+ * Autotune times every shape (reps runs each), keeps the fastest and returns
+ * its time per run in microseconds. */
+void   cvdvBatchSetLaunch(CVDVBatch* batch, int threads, int ldsBytes);
+void   cvdvBatchGetLaunch(CVDVBatch* batch, int* threads, int* ldsBytes);
+double cvdvBatchAutotune(CVDVBatch* batch, int reps);
+void   cvdvBatchRun(CVDVBatch* batch);
+void   cvdvBatchSynchronize(CVDVBatch* batch);
+void   cvdvBatchMeasure(CVDVBatch* batch, int regIdx, double* probsOut);
+void   cvdvBatchNorms(CVDVBatch* batch, double* normsOut);
+void   cvdvBatchGetState(CVDVBatch* batch, int i, double* realOut, double* imagOut);
+
 #ifdef __cplusplus
 }
 #endif

@@ -118,6 +118,7 @@ static void ctx_release(CVDVContext* ctx) {
     ctx->arenaBytes = 0;
     if (ctx->tw) twiddle_cache_free(ctx->tw);
     ctx->tw = nullptr;
+    blas_release(ctx);
     ctx->initialized = false;
 }
 

@@ -193,6 +193,7 @@ struct CVDVContext {
     void* arena = nullptr;            // phase tables of the compiled pending program
     size_t arenaBytes = 0;
     BatchRecording* rec = nullptr;    // non-null: record ops, launch nothing
+    void* blas = nullptr;             // rocBLAS handle of the grid readouts (created on first use)
     bool profOn = false;
     std::vector<ProfRec> prof;        // recorded launches
     std::vector<hipEvent_t> evPool;   // reusable events
@@ -240,6 +241,8 @@ void* ctx_scratch(CVDVContext* ctx, size_t bytes);
 
 // batch recording (gates.hip): append an op whose phase program is `p`
 void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p);
+// rocBLAS handle of the grid readouts (phase_grid.hip)
+void blas_release(CVDVContext* ctx);
 // readouts refuse recording contexts (no state of their own)
 bool refuse_recording(CVDVContext* ctx);
 // Stockham stage twiddles of length 2^logM in the given precision (fft.hip):

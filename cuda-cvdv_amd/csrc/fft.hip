@@ -218,37 +218,50 @@ size_t device_max_lds() {
 
 // ---------------------------------------------------------------- pass descriptor
 struct FftPass {
-    int lgM, lgC, lgHb;            // tile = [Hb][M][C]
+    int lgM, lgC, lgHb;            // tile = [Hb][M][C] (lgM also selects the kernel instance)
     int lgNLoBlk, lgNH2Blk;        // grid = nH1 * nH2Blk * nLoBlk
-    int rowFastIn, rowFastOut;     // linear order after the column index
-    int vecIn, vecOut;             // 1: pairs of complex64 move as one 16-byte access
+    int rowFastOut;                // linear order of the staged store after the column index
+    int vecOut;                    // 1: pairs of complex64 stored as one 16-byte access
     int directOut;                 // 1: last stage stores straight to global (coalesced rows)
-    int nst;
-    int rad[6], lgL[6], off[6];
     int tw4, tw4Shift, tw4LogN;
-    int usePre;
+    int usePre, useScale;
+    int conjIn, conjOut;           // inverse = conj . forward . conj, split over the passes
     int lgInH1, lgInH2, lgInR;     // every stride is a power of two
     int lgOutH1, lgOutH2, lgOutR;
-    double outScale;  // applied as hi + lo parts (FMA) so an FP32 store keeps it unbiased
+    double outScale;  // applied as exact v*hi + rounded v*lo (one FMA): an FP32 store stays unbiased
 };
 
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
+// Stage plan of a length-2^LGM transform: radix-8 stages, then one radix-4 or
+// radix-2 stage.  Stage s has sub-length L = 8^s and its twiddles start at
+// entry off(s) of the stage table (W_{RL}^{ij} at off + j (R-1) + i - 1), the
+// layout stage_values() writes.
+template <int LGM> struct Plan {
+    static constexpr int NST = LGM / 3 + (LGM % 3 ? 1 : 0);
+    static constexpr int lgr(int s) { return s < LGM / 3 ? 3 : LGM % 3; }
+    static constexpr int off(int s) {
+        int o = 0, L = 1;
+        for (int i = 0; i < s; ++i) {
+            o += L * ((1 << lgr(i)) - 1);
+            L <<= lgr(i);
+        }
+        return o;
+    }
+};
+
 // ---------------------------------------------------------------- butterflies (fft_bfly.h)
 
 // Tile element e (linear) -> (c, row, hb) in the given order.
-__device__ __forceinline__ void tile_coords(int e, int lgC, int lgM, int lgHb, int rowFast, int& c, int& row,
-                                            int& hb) {
+template <int LGM>
+__device__ __forceinline__ void tile_coords(int e, int lgC, int lgHb, int rowFast, int& c, int& row, int& hb) {
     c = e & ((1 << lgC) - 1);
     const int rest = e >> lgC;
-    if (rowFast) {
-        row = rest & ((1 << lgM) - 1);
-        hb = rest >> lgM;
-    } else {
-        hb = rest & ((1 << lgHb) - 1);
-        row = rest >> lgHb;
-    }
+    const int lgLo = rowFast ? LGM : lgHb;  // selects, not branches on references (no scratch)
+    const int lo = rest & ((1 << lgLo) - 1), hi = rest >> lgLo;
+    row = rowFast ? lo : hi;
+    hb = rowFast ? hi : lo;
 }
 
 // Stockham stage geometry.  Butterfly b of a stage of radix RP (sub-length
@@ -259,10 +272,10 @@ struct BflyPos {
     int c, q, hb;
 };
 
-template <int RP>
-__device__ __forceinline__ BflyPos bfly_pos(int b, int lgC, int lgM) {
+template <int RP, int LGM>
+__device__ __forceinline__ BflyPos bfly_pos(int b, int lgC) {
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgMq = lgM - LGR;
+    constexpr int lgMq = LGM - LGR;
     BflyPos p;
     p.c = b & ((1 << lgC) - 1);
     const int rest = b >> lgC;
@@ -271,123 +284,182 @@ __device__ __forceinline__ BflyPos bfly_pos(int b, int lgC, int lgM) {
     return p;
 }
 
-// Twiddle the inputs of the butterflies in x (stage s, radix RP) and transform.
-template <int RP, int E, bool INV, typename C>
-__device__ __forceinline__ void stage_compute(C* x, const FftPass& d, int s, const C* __restrict__ tw,
-                                              int tid, int nt) {
+// Twiddle the inputs of the butterflies in x (stage S, radix RP) and transform.
+template <int RP, int E, int LGM, int S, typename C>
+__device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, int lgC, int tid, int nt) {
     constexpr int NB = E / RP;
-    const int lgL = d.lgL[s];
+    constexpr int lgL = 3 * S;
+    constexpr int OFF = Plan<LGM>::off(S);
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         if (lgL > 0) {
-            const BflyPos p = bfly_pos<RP>(tid + k * nt, d.lgC, d.lgM);
-            const C* t = tw + d.off[s] + (p.q & ((1 << lgL) - 1)) * (RP - 1);
+            const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
+            const C* t = tw + OFF + (p.q & ((1 << lgL) - 1)) * (RP - 1);
 #pragma unroll
-            for (int i = 1; i < RP; ++i) x[k * RP + i] = mul_w<INV>(x[k * RP + i], t[i - 1]);
+            for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
         }
-        dft_r<RP, INV>(x + k * RP);
+        dft_r<RP, false>(x + k * RP);
     }
 }
 
-// Outputs of stage s (radix RP) -> LDS in Stockham order.
-template <int RP, int E, typename C>
-__device__ __forceinline__ void stage_write(C* lds, const C* x, const FftPass& d, int s, int tid, int nt) {
+// Outputs of stage S (radix RP) -> LDS in Stockham order.
+template <int RP, int E, int LGM, int S, typename C>
+__device__ __forceinline__ void stage_write(C* lds, const C* x, int lgC, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgC = d.lgC, lgM = d.lgM, lgL = d.lgL[s];
+    constexpr int lgL = 3 * S;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
-        const int ob = (p.hb << lgM) + ((p.q >> lgL) << (LGR + lgL)) + (p.q & ((1 << lgL) - 1));
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
+        const int ob = (p.hb << LGM) + ((p.q >> lgL) << (LGR + lgL)) + (p.q & ((1 << lgL) - 1));
 #pragma unroll
         for (int m = 0; m < RP; ++m) lds[lds_ix(((ob + (m << lgL)) << lgC) + p.c)] = x[k * RP + m];
     }
 }
 
-// Inputs of stage s (radix RP) <- LDS.
-template <int RP, int E, typename C>
-__device__ __forceinline__ void stage_read(const C* lds, C* x, const FftPass& d, int tid, int nt) {
+// Inputs of a stage of radix RP <- LDS.
+template <int RP, int E, int LGM, typename C>
+__device__ __forceinline__ void stage_read(const C* lds, C* x, int lgC, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+    constexpr int lgMq = LGM - LGR;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
-        const int base = (p.hb << lgM) + p.q;
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
+        const int base = (p.hb << LGM) + p.q;
 #pragma unroll
         for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_ix(((base + (i << lgMq)) << lgC) + p.c)];
     }
 }
 
+// The compiled pending program, staged in LDS so its runtime-length loops read
+// LDS instead of forcing a scratch copy of the kernel argument.
+struct PreLds {
+    DiagTab tab[CVDV_FAST_TABS];
+    DiagTerm gen[CVDV_FAST_GEN];
+};
+
+// conj(v * f) when cj, else v * f (inverse transforms conjugate their input).
+template <typename C>
+__device__ __forceinline__ C mul_conj_sel(C v, C f, int cj) {
+    C r = cmul(v, f);
+    if (cj) r.y = -r.y;
+    return r;
+}
+
+template <typename C>
+__device__ __forceinline__ C pre_factor(const PreLds& P, const FastDiag& pre, uint64_t g) {
+    using R = decltype(C{}.x);
+    C fac = cmk<C>(1.0, 0.0);
+    if (pre.ntab > 0) fac = tab_entry<C>(pre.arena, P.tab[0], g);
+    for (int i = 1; i < pre.ntab; ++i) fac = cmul(fac, tab_entry<C>(pre.arena, P.tab[i], g));
+    if (pre.ngen > 0) {
+        double r = pre.genrev;
+        for (int i = 0; i < pre.ngen; ++i) r += cvdv_frac(diag_term_rev(P.gen[i], g));
+        const C u = unit_rev_fast(cvdv_frac(r), R(0));
+        fac = pre.ntab > 0 ? cmul(fac, u) : u;
+    }
+    return fac;
+}
+
 // Stage-0 inputs straight from global memory, pending phases applied.
-template <int RP, int E, typename C>
+template <int RP, int E, int LGM, typename C>
 __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
-                                            const FastDiag& pre, int tid, int nt) {
+                                            const PreLds& P, const FastDiag& pre, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+    constexpr int lgMq = LGM - LGR;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, d.lgC);
         const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
 #pragma unroll
         for (int i = 0; i < RP; ++i) {
             const uint64_t g = g0 + ((uint64_t)(i << lgMq) << d.lgInR);
-            x[k * RP + i] = d.usePre ? apply_fast(pre, g, in[g]) : in[g];
+            C v = in[g];
+            if (d.usePre) v = mul_conj_sel(v, pre_factor<C>(P, pre, g), d.conjIn);
+            else if (d.conjIn) v.y = -v.y;
+            x[k * RP + i] = v;
         }
     }
 }
 
-template <bool INV, typename R, typename C>
+template <typename R, typename C>
 __device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int row, const C* __restrict__ tw4,
                                         R scale, R scaleLo) {
     if (d.tw4) {
         const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
-        v = mul_w<INV>(v, tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)]);
+        v = cmul(v, tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)]);
     }
-    v.x = fma(v.x, scale, v.x * scaleLo);  // exact v*hi + rounded v*lo, one rounding
-    v.y = fma(v.y, scale, v.y * scaleLo);
+    if (d.conjOut) v.y = -v.y;
+    if (d.useScale) {
+        v.x = fma(v.x, scale, v.x * scaleLo);
+        v.y = fma(v.y, scale, v.y * scaleLo);
+    }
     return v;
 }
 
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
-template <int RP, int E, bool INV, typename R, typename C>
+template <int RP, int E, int LGM, typename R, typename C>
 __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
                                            uint64_t lo0, const C* __restrict__ tw4, R scale, R scaleLo, int tid,
                                            int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    const int lgC = d.lgC, lgM = d.lgM, lgMq = lgM - LGR;
+    constexpr int lgMq = LGM - LGR;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP>(tid + k * nt, lgC, lgM);
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, d.lgC);
         const uint64_t g0 = outBase + ((uint64_t)p.hb << d.lgOutH2) + p.c;
 #pragma unroll
         for (int m = 0; m < RP; ++m) {
             const int row = p.q + (m << lgMq);
-            out[g0 + ((uint64_t)row << d.lgOutR)] =
-                out_finish<INV>(x[k * RP + m], d, lo0 + p.c, row, tw4, scale, scaleLo);
+            out[g0 + ((uint64_t)row << d.lgOutR)] = out_finish(x[k * RP + m], d, lo0 + p.c, row, tw4, scale, scaleLo);
         }
     }
 }
 
-#define CVDV_RADIX_SWITCH(RPV, CALL)                    \
-    do {                                                \
-        if ((RPV) == 8) { constexpr int RP = 8; CALL; } \
-        else if ((RPV) == 4) { constexpr int RP = 4; CALL; } \
-        else { constexpr int RP = 2; CALL; }            \
-    } while (0)
+// Stages 1 .. NST-1: exchange the previous stage's outputs through LDS, then
+// twiddle and transform.
+template <int LGM, int S, int E, typename C>
+__device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ tw, int lgC, int tid, int nt) {
+    if constexpr (S < Plan<LGM>::NST) {
+        constexpr int RPprev = 1 << Plan<LGM>::lgr(S - 1);
+        constexpr int RP = 1 << Plan<LGM>::lgr(S);
+        stage_write<RPprev, E, LGM, S - 1>(lds, x, lgC, tid, nt);
+        __syncthreads();
+        stage_read<RP, E, LGM>(lds, x, lgC, tid, nt);
+        __syncthreads();
+        stage_compute<RP, E, LGM, S>(x, tw, lgC, tid, nt);
+        run_stages<LGM, S + 1, E>(x, lds, tw, lgC, tid, nt);
+    }
+}
 
-template <typename R, bool INV, int E>
+template <typename R, int LGM, int E>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
            const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
     using C = typename CT<R>::type;
+    constexpr int NST = Plan<LGM>::NST;
+    constexpr int R0 = 1 << Plan<LGM>::lgr(0);
+    constexpr int RL = 1 << Plan<LGM>::lgr(NST - 1);
     extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ PreLds P;
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lgM = d.lgM, lgC = d.lgC, lgHb = d.lgHb;
-    const int tile = 1 << (lgM + lgC + lgHb);
+    const int lgC = d.lgC, lgHb = d.lgHb;
+    const int tile = 1 << (LGM + lgC + lgHb);
+
+    if (d.usePre) {
+        // constant indices: the argument is read in place
+#pragma unroll
+        for (int i = 0; i < CVDV_FAST_TABS; ++i)
+            if (tid == i && i < pre.ntab) P.tab[i] = pre.tab[i];
+#pragma unroll
+        for (int i = 0; i < CVDV_FAST_GEN; ++i)
+            if (tid == CVDV_FAST_TABS + i && i < pre.ngen) P.gen[i] = pre.gen[i];
+        __syncthreads();
+    }
 
     uint64_t b = blockIdx.x;
     const uint64_t loBlk = b & ((1ull << d.lgNLoBlk) - 1);
@@ -401,43 +473,34 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     const R scaleLo = (R)(d.outScale - (double)scale);
 
     C x[E];
-    // stage 0: loads of the first butterflies, phases, first transform (no twiddles)
-    CVDV_RADIX_SWITCH(d.rad[0], (stage0_load<RP, E>(in, x, d, inBase, pre, tid, nt),
-                                 stage_compute<RP, E, INV>(x, d, 0, tw, tid, nt)));
-    for (int s = 1; s < d.nst; ++s) {
-        CVDV_RADIX_SWITCH(d.rad[s - 1], (stage_write<RP, E>(lds, x, d, s - 1, tid, nt)));
-        __syncthreads();
-        CVDV_RADIX_SWITCH(d.rad[s], (stage_read<RP, E>(lds, x, d, tid, nt)));
-        __syncthreads();
-        CVDV_RADIX_SWITCH(d.rad[s], (stage_compute<RP, E, INV>(x, d, s, tw, tid, nt)));
-    }
-    const int last = d.nst - 1;
+    stage0_load<R0, E, LGM>(in, x, d, inBase, P, pre, tid, nt);
+    stage_compute<R0, E, LGM, 0>(x, tw, lgC, tid, nt);
+    run_stages<LGM, 1, E>(x, lds, tw, lgC, tid, nt);
     if (d.directOut) {
-        CVDV_RADIX_SWITCH(d.rad[last], (last_store<RP, E, INV>(out, x, d, outBase, lo0, tw4, scale, scaleLo,
-                                                               tid, nt)));
+        last_store<RL, E, LGM>(out, x, d, outBase, lo0, tw4, scale, scaleLo, tid, nt);
         return;
     }
     // staged store: natural order through LDS, written in the output's contiguous order
-    CVDV_RADIX_SWITCH(d.rad[last], (stage_write<RP, E>(lds, x, d, last, tid, nt)));
+    stage_write<RL, E, LGM, NST - 1>(lds, x, lgC, tid, nt);
     __syncthreads();
     if (sizeof(C) == 8 && d.vecOut) {
         for (int e = 2 * tid; e < tile; e += 2 * nt) {
             int c, row, hb, c1, row1, hb1;
-            tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
-            tile_coords(e + 1, lgC, lgM, lgHb, d.rowFastOut, c1, row1, hb1);
-            const C v0 = out_finish<INV>(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], d, lo0 + c, row, tw4,
-                                         scale, scaleLo);
-            const C v1 = out_finish<INV>(lds[lds_ix((((hb1 << lgM) + row1) << lgC) + c1)], d, lo0 + c1, row1,
-                                         tw4, scale, scaleLo);
+            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+            tile_coords<LGM>(e + 1, lgC, lgHb, d.rowFastOut, c1, row1, hb1);
+            const C v0 = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4, scale,
+                                    scaleLo);
+            const C v1 = out_finish(lds[lds_ix((((hb1 << LGM) + row1) << lgC) + c1)], d, lo0 + c1, row1, tw4,
+                                    scale, scaleLo);
             const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
             *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
         }
     } else {
         for (int e = tid; e < tile; e += nt) {
             int c, row, hb;
-            tile_coords(e, lgC, lgM, lgHb, d.rowFastOut, c, row, hb);
-            const C v = out_finish<INV>(lds[lds_ix((((hb << lgM) + row) << lgC) + c)], d, lo0 + c, row, tw4,
-                                        scale, scaleLo);
+            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+            const C v = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4, scale,
+                                   scaleLo);
             out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = v;
         }
     }
@@ -499,7 +562,6 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     auto lg0 = [](uint64_t v) { return v ? ilog2(v) : 0; };  // stride 0: that index is always 0
     d.lgInH1 = lg0(g.inH1); d.lgInH2 = lg0(g.inH2); d.lgInR = lg0(g.inR);
     d.lgOutH1 = lg0(g.outH1); d.lgOutH2 = lg0(g.outH2); d.lgOutR = lg0(g.outR);
-    d.rowFastIn = g.inR <= g.inH2 ? 1 : 0;
     d.rowFastOut = g.outR <= g.outH2 ? 1 : 0;
     // 16-byte pairs along the fastest dimension when it is contiguous in memory
     auto vec_ok = [&](int rowFast, uint64_t sR, uint64_t sH) {
@@ -508,7 +570,6 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
         if (rowFast) return (sR == 1 && g.M >= 2) ? 1 : 0;
         return (sH == 1 && Hb >= 2) ? 1 : 0;
     };
-    d.vecIn = vec_ok(d.rowFastIn, g.inR, g.inH2);
     d.vecOut = vec_ok(d.rowFastOut, g.outR, g.outH2);
     // last stage writes rows q + m*M/R of column c: coalesced when a full
     // column group is one 128-byte run or when rows follow the columns directly
@@ -516,39 +577,45 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     return true;
 }
 
+template <typename R, int LGM>
+static void launch_lgm(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
+                       const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
+                       const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
+    auto go = [&](auto kfn, int E) {
+        if (lds > 64 * 1024)
+            HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+        hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw, tw4);
+    };
+    if constexpr (LGM == 14 && sizeof(R) == 4) {  // 2^14 complex128 never fits one workgroup's LDS
+        if (tile / 8 > 1024) {
+            go(k_fft_tile<R, LGM, 16>, 16);
+            return;
+        }
+    }
+    go(k_fft_tile<R, LGM, 8>, 8);
+}
+
 template <typename R>
 static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
-                        FftPass d, const FastDiag& pre, bool inverse, int prec, uint64_t nH1) {
+                        const FftPass& d, const FastDiag& pre, int prec, uint64_t nH1) {
     using C = typename CT<R>::type;
-    const StagePlan sp = stage_plan(d.lgM);
-    d.nst = sp.nst;
-    for (int s = 0; s < sp.nst; ++s) {
-        d.rad[s] = sp.rad[s];
-        d.lgL[s] = ilog2((uint64_t)sp.L[s]);
-        d.off[s] = sp.off[s];
-    }
     const C* tw = (const C*)stage_table(ctx, d.lgM, prec);
     const C* tw4 = d.tw4 ? (const C*)full_table(ctx, d.tw4LogN, prec) : nullptr;
     const int tile = 1 << (d.lgM + d.lgC + d.lgHb);
-    const int E = tile / 8 > 1024 ? 16 : 8;
-    const int nt = tile / E;
+    if (tile / 16 > 1024 || (tile / 8 > 1024 && d.lgM != 14))
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile larger than one workgroup"};
     const uint64_t blocks = nH1 << (d.lgNH2Blk + d.lgNLoBlk);
     const size_t lds = lds_bytes(tile, sizeof(C));
     const int pid = prof_begin(ctx, KK_FFT);
-#define CVDV_FFT_LAUNCH(INVV, EE)                                                                     \
-    do {                                                                                             \
-        auto kfn = k_fft_tile<R, INVV, EE>;                                                          \
-        if (lds > 64 * 1024)                                                                         \
-            HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, \
-                                       (int)lds));                                                   \
-        hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(nt), lds, ctx->stream, in, out, d, pre, tw, tw4); \
-    } while (0)
-    if (inverse) {
-        if (E == 16) CVDV_FFT_LAUNCH(true, 16); else CVDV_FFT_LAUNCH(true, 8);
-    } else {
-        if (E == 16) CVDV_FFT_LAUNCH(false, 16); else CVDV_FFT_LAUNCH(false, 8);
+    switch (d.lgM) {
+#define CVDV_LGM_CASE(L) \
+    case L: launch_lgm<R, L>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        CVDV_LGM_CASE(1) CVDV_LGM_CASE(2) CVDV_LGM_CASE(3) CVDV_LGM_CASE(4) CVDV_LGM_CASE(5)
+        CVDV_LGM_CASE(6) CVDV_LGM_CASE(7) CVDV_LGM_CASE(8) CVDV_LGM_CASE(9) CVDV_LGM_CASE(10)
+        CVDV_LGM_CASE(11) CVDV_LGM_CASE(12) CVDV_LGM_CASE(13) CVDV_LGM_CASE(14)
+#undef CVDV_LGM_CASE
+        default: throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile length out of range"};
     }
-#undef CVDV_FFT_LAUNCH
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
 }
@@ -561,9 +628,10 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     using C = typename CT<R>::type;
     const uint64_t N = 1ull << nb, S = 1ull << sh;
     const uint64_t H = total / (N * S);
-    bool act;
-    const FastDiag pre = compile_diag(ctx, preProg, act);
+    bool act = false, folded = false;
+    const FastDiag pre = compile_diag(ctx, preProg, act, outScale, &folded);
     const int usePre = act ? 1 : 0;
+    const int inv = inverse ? 1 : 0;
 
     // single pass: rows j (stride S), lo in [0,S), hi blocks in h2 (stride N*S)
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
@@ -571,22 +639,26 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     if (fit_pass(g1, prec, d, true)) {
         d.tw4 = 0; d.tw4Shift = 0; d.tw4LogN = 0;
         d.usePre = usePre;
+        d.useScale = folded ? 0 : 1;
+        d.conjIn = inv;
+        d.conjOut = inv;
         d.outScale = outScale;
-        launch_pass<R>(ctx, data, data, d, pre, inverse, prec, 1);
+        launch_pass<R>(ctx, data, data, d, pre, prec, 1);
         return false;
     }
     if (total < 4096 || N <= 64) {
-        // tiny state: direct DFT, out of place
+        // tiny state: direct DFT, out of place, FP64 accumulation rounded once
+        const double sc = folded ? 1.0 : outScale;
         const double2* twN = (const double2*)full_table(ctx, nb);
         C* tmp = tmpFn();
         uint64_t blocks = (total + 255) / 256;
         const int pid = prof_begin(ctx, KK_FFT);
         if (inverse)
             hipLaunchKernelGGL((k_dft_naive<R, true>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               data, tmp, total, sh, nb, pre, usePre, outScale, twN);
+                               data, tmp, total, sh, nb, pre, usePre, sc, twN);
         else
             hipLaunchKernelGGL((k_dft_naive<R, false>), dim3((unsigned)blocks), dim3(256), 0, ctx->stream,
-                               data, tmp, total, sh, nb, pre, usePre, outScale, twN);
+                               data, tmp, total, sh, nb, pre, usePre, sc, twN);
         HIPCHK(hipGetLastError());
         prof_end(ctx, pid);
         return true;
@@ -602,15 +674,22 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     FftPass d2{};
     if (!fit_pass(p1, prec, d1, false) || !fit_pass(p2, prec, d2, false))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
+    // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its output
     d1.tw4 = 1; d1.tw4Shift = sh; d1.tw4LogN = nb;
     d1.usePre = usePre;
+    d1.useScale = 0;
+    d1.conjIn = inv;
+    d1.conjOut = 0;
     d1.outScale = 1.0;
     d2.tw4 = 0; d2.tw4Shift = 0; d2.tw4LogN = 0;
     d2.usePre = 0;
+    d2.useScale = folded ? 0 : 1;
+    d2.conjIn = 0;
+    d2.conjOut = inv;
     d2.outScale = outScale;
     C* tmp = tmpFn();
-    launch_pass<R>(ctx, data, data, d1, pre, inverse, prec, H);
-    launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, inverse, prec, H);
+    launch_pass<R>(ctx, data, data, d1, pre, prec, H);
+    launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, prec, H);
     return true;
 }
 

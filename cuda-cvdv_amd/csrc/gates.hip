@@ -26,12 +26,27 @@ __global__ void k_diag(C* __restrict__ s, uint64_t total, FastDiag f) {
         s[g] = apply_fast(f, g, s[g]);
 }
 
+// A whole program evaluated per element in FP64 (programs beyond the compiled
+// capacities; rare).  The program sits in device memory: a kernel-argument
+// copy indexed by a runtime term count would be spilled to scratch.
+template <typename C>
+__global__ void k_diag_prog(C* __restrict__ s, uint64_t total, const DiagProg* __restrict__ p) {
+    using R = decltype(C{}.x);
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x) {
+        double r = p->grev;
+        for (int i = 0; i < p->n; ++i) r += cvdv_frac(diag_term_rev(p->t[i], g));
+        s[g] = cmul(s[g], unit_rev(cvdv_frac(r), R(0)));
+    }
+}
+
 struct TabMake {
     int kind, nb;
     uint32_t off;
     int pad;
     double c1, c2, c3;
-    double g;  // constant phase folded into this table (revolutions)
+    double g;      // constant phase folded into this table (revolutions)
+    double scale;  // constant magnitude folded into this table
 };
 
 struct TabMakeArgs {
@@ -57,7 +72,7 @@ __global__ void k_make_tables(C* __restrict__ arena, TabMakeArgs a) {
             v = j ? t.c1 : -t.c1;
         }
         const double2 u = unit_rev(cvdv_frac(cvdv_frac(v) + t.g), 0.0);
-        arena[e] = cmk<C>(u.x, u.y);
+        arena[e] = cmk<C>(u.x * t.scale, u.y * t.scale);
     }
 }
 
@@ -130,41 +145,89 @@ DiagProg ctx_take_pending(CVDVContext* ctx) {
     return p;
 }
 
-FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active) {
+static bool term_is_identity(const DiagTerm& t) {
+    return t.c1 == 0.0 && t.c2 == 0.0 && t.c3 == 0.0;
+}
+
+// The whole program as one FP64 pass over the state (programs that exceed the
+// compiled capacities).
+static void diag_prog_pass(CVDVContext* ctx, const DiagProg& p) {
+    const size_t need = sizeof(DiagProg);
+    DiagProg* d = (DiagProg*)ctx_scratch(ctx, need);
+    HIPCHK(hipMemcpyAsync(d, &p, need, hipMemcpyHostToDevice, ctx->stream));
+    const uint64_t n = ctx->totalSize;
+    const int pid = prof_begin(ctx, KK_DIAG);
+    if (ctx->prec == CVDV_C128)
+        hipLaunchKernelGGL(k_diag_prog<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (double2*)ctx->state(), n, (const DiagProg*)d);
+    else
+        hipLaunchKernelGGL(k_diag_prog<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (float2*)ctx->state(), n, (const DiagProg*)d);
+    HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
+    // the host copy must outlive the asynchronous transfer
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale, bool* scaleFolded) {
     FastDiag f{};
-    active = prog_active(p);
-    const double ga = 2.0 * PI * p.grev;
-    f.gphase = make_double2(std::cos(ga), std::sin(ga));
-    if (!active) return f;
+    if (scaleFolded) *scaleFolded = false;
+    active = false;
     TabMakeArgs a{};
     uint32_t off = 0;
+    int ngen = 0;
     for (int i = 0; i < p.n; ++i) {
         const DiagTerm& t = p.t[i];
+        if (term_is_identity(t)) continue;
         const bool tab = (t.kind == DT_POLY && t.nb1 <= 20) || t.kind == DT_ZBIT;
-        if (!tab) {
-            f.gen.t[f.gen.n++] = t;
-            continue;
+        if (tab) {
+            if (f.ntab == CVDV_FAST_TABS) { ngen = -1; break; }
+            const int nb = t.kind == DT_ZBIT ? 1 : t.nb1;
+            f.tab[f.ntab++] = DiagTab{t.sh1, nb, t.ctrl, off};
+            a.t[a.n++] = TabMake{t.kind, nb, off, 0, t.c1, t.c2, t.c3, 0.0, 1.0};
+            off += 1u << nb;
+        } else {
+            if (ngen == CVDV_FAST_GEN) { ngen = -1; break; }
+            f.gen[ngen++] = t;
         }
-        const int nb = t.kind == DT_ZBIT ? 1 : t.nb1;
-        f.tab[f.ntab++] = DiagTab{t.sh1, nb, t.ctrl, off};
-        a.t[a.n++] = TabMake{t.kind, nb, off, 0, t.c1, t.c2, t.c3};
-        off += 1u << nb;
     }
-    f.usegen = f.gen.n > 0;
-    // The global phase is folded into an FP64-evaluated angle (an unconditional
-    // table, else the per-element program) instead of a constant factor rounded
-    // to FP32, whose |g|^2 - 1 would scale the norm by the same amount on every pass.
-    bool folded = false;
-    for (int i = 0; i < a.n && !folded; ++i)
+    if (ngen < 0) {
+        // beyond the compiled capacities: one FP64 pass now, nothing left to apply
+        diag_prog_pass(ctx, p);
+        return FastDiag{};
+    }
+    f.ngen = ngen;
+    // The global phase and the scale are folded into an unconditional table
+    // (exact in FP64 before the single rounding); a rounded constant factor
+    // would scale the norm by the same |g|^2 - 1 on every pass.  Without such a
+    // table they go to the generic terms' angle (phase only) or to a one-entry
+    // table.
+    const double grev = cvdv_frac(p.grev);
+    int host = -1;
+    for (int i = 0; i < a.n; ++i)
         if (f.tab[i].ctrl < 0) {
-            a.t[i].g = cvdv_frac(p.grev);
-            folded = true;
+            host = i;
+            break;
         }
-    if (!folded && f.usegen) {
-        f.gen.grev = cvdv_frac(p.grev);
-        folded = true;
+    if (host < 0 && (grev != 0.0 || scale != 1.0)) {
+        if (scale == 1.0 && f.ngen > 0) {
+            f.genrev = grev;
+        } else if (f.ntab < CVDV_FAST_TABS) {
+            host = f.ntab;
+            f.tab[f.ntab++] = DiagTab{0, 0, -1, off};
+            a.t[a.n++] = TabMake{DT_POLY, 0, off, 0, 0.0, 0.0, 0.0, 0.0, 1.0};
+            off += 1;
+        } else {
+            diag_prog_pass(ctx, p);
+            return FastDiag{};
+        }
     }
-    if (folded) f.gphase = make_double2(1.0, 0.0);
+    if (host >= 0) {
+        a.t[host].g = grev;
+        a.t[host].scale = scale;
+        if (scaleFolded) *scaleFolded = true;
+    }
+    active = f.ntab > 0 || f.ngen > 0;
     if (a.n) {
         a.total = off;
         const size_t need = (size_t)off * ctx->elemBytes();
@@ -188,6 +251,36 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active) {
     }
     f.arena = ctx->arena;
     return f;
+}
+
+// Bits of the flat index a term depends on.
+static uint64_t term_bits(const DiagTerm& t) {
+    auto field = [](int sh, int nb) { return (((uint64_t)1 << nb) - 1ull) << sh; };
+    uint64_t m;
+    if (t.kind == DT_POLY) m = field(t.sh1, t.nb1);
+    else if (t.kind == DT_BILIN) m = field(t.sh1, t.nb1) | field(t.sh2, t.nb2);
+    else m = 1ull << t.sh1;
+    if (t.ctrl >= 0) m |= 1ull << t.ctrl;
+    return m;
+}
+
+// The pending terms that do not commute with an operation mixing the index
+// bits in `mix` (an FFT along a field, a 2x2 gate on a bit, a conditional
+// reflection of a field).  A diagonal term that depends on none of those bits
+// has the same value on every element the operation mixes, so it stays
+// pending; the global phase goes with any non-empty taken program.
+static DiagProg ctx_take_involving(CVDVContext* ctx, uint64_t mix) {
+    DiagProg take{}, keep{};
+    for (int i = 0; i < ctx->pending.n; ++i) {
+        const DiagTerm& t = ctx->pending.t[i];
+        if (term_is_identity(t)) continue;
+        if (term_bits(t) & mix) take.t[take.n++] = t;
+        else keep.t[keep.n++] = t;
+    }
+    if (take.n > 0) take.grev = ctx->pending.grev;
+    else keep.grev = ctx->pending.grev;
+    ctx->pending = keep;
+    return take;
 }
 
 void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p) {
@@ -328,7 +421,7 @@ void ctx_ft(CVDVContext* ctx, int reg, bool inverse) {
     tw.nb1 = nb;
     tw.ctrl = -1;
     tw.c1 = sgn * (double)(N - 1) / (4.0 * (double)N);
-    DiagProg pre = ctx_take_pending(ctx);
+    DiagProg pre = ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[reg]);
     if (pre.n == CVDV_MAX_TERMS) {
         bool found = false;
         for (int i = 0; i < pre.n; ++i) found |= same_slot(pre.t[i], tw);
@@ -349,10 +442,9 @@ void ctx_ft(CVDVContext* ctx, int reg, bool inverse) {
     } else {
         fft_register(ctx, ctx->pshift[reg], nb, inverse, pre, 1.0 / std::sqrt((double)N));
     }
-    ctx->pending = DiagProg{};
-    prog_add(ctx->pending, tw);
     const double gphase = (double)(N - 1) * (double)(N - 1) / (4.0 * (double)N);
-    ctx->pending.grev = cvdv_frac(sgn * gphase);
+    ctx->pending.grev = cvdv_frac(ctx->pending.grev + sgn * gphase);
+    ctx_push_term(ctx, tw);
 }
 
 static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) {
@@ -369,11 +461,11 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
         op.kind = BOP_QUBIT;
         op.bit = bit;
         for (int i = 0; i < 4; ++i) op.m[i] = make_double2(mm[2 * i], mm[2 * i + 1]);
-        rec_op(ctx, op, ctx_take_pending(ctx));
+        rec_op(ctx, op, ctx_take_involving(ctx, 1ull << bit));
         return;
     }
     bool act;
-    const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
+    const FastDiag p = compile_diag(ctx, ctx_take_involving(ctx, 1ull << bit), act);
     const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const double2 m00 = make_double2(mm[0], mm[1]), m01 = make_double2(mm[2], mm[3]);
@@ -436,11 +528,12 @@ static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
         op.bit = cpos;
         op.ctrlFlip = ctx->refl[creg];
         op.tgtRefl = ctx->refl[tgt];
-        rec_op(ctx, op, ctx_take_pending(ctx));
+        rec_op(ctx, op, ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]));
         return;
     }
     bool act;
-    const FastDiag p = compile_diag(ctx, ctx_take_pending(ctx), act);
+    const FastDiag p =
+        compile_diag(ctx, ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]), act);
     const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const int pid = prof_begin(ctx, KK_CPARITY);

@@ -68,6 +68,10 @@ SIGNATURES = {
     "cvdvProfileEnable": ([c_void_p, c_int], None),
     "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),
     "cvdvProfileReset": ([c_void_p], None),
+    "cvdvGetWignerGrid": ([c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_double), c_int, c_int, c_int,
+                           POINTER(c_double)], None),
+    "cvdvGetHusimiQGrid": ([c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_double), c_int, c_int,
+                            POINTER(c_double)], None),
     "cvdvBatchCreate": ([c_int, POINTER(c_int), POINTER(c_int), c_int], c_void_p),
     "cvdvBatchDestroy": ([c_void_p], None),
     "cvdvBatchNumCircuits": ([c_void_p], c_int),

@@ -263,6 +263,33 @@ class HipCvdv:
         return out.reshape((dim, dim))
 
     # ------------------------------------------------------------ additions
+    def _grid(self, name, regIdx, a, b, per_slice, *extra):
+        a = np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(-1))
+        b = np.ascontiguousarray(np.asarray(b, dtype=np.float64).reshape(-1))
+        if not 0 <= regIdx < self.num_registers:
+            raise ValueError(f"Invalid register index: {regIdx}")
+        S = self.total_size // self.register_dims[regIdx]
+        blocks = S if per_slice else 1
+        out = np.zeros(blocks * b.size * a.size, dtype=np.float64)
+        self._c(name, c_int(regIdx), a.ctypes.data_as(POINTER(c_double)), c_int(a.size),
+                b.ctypes.data_as(POINTER(c_double)), c_int(b.size), c_int(1 if per_slice else 0), *extra,
+                out.ctypes.data_as(POINTER(c_double)))
+        shape = (blocks, b.size, a.size) if per_slice else (b.size, a.size)
+        return out.reshape(shape)
+
+    def getWignerGrid(self, regIdx: int, xs, ps, per_slice: bool = False,
+                      native: bool = True) -> npt.NDArray[np.float64]:
+        """Wigner of register regIdx at the points (xs, ps): shape (len(ps), len(xs)), or
+        (S, len(ps), len(xs)) per slice of the other registers (canonical order).
+        native=True equals getWigner() on its grid (including its e^{-i(N-1)dx p}
+        finalisation); native=False drops that phase (the symmetric sum)."""
+        return self._grid("cvdvGetWignerGrid", regIdx, xs, ps, per_slice, c_int(1 if native else 0))
+
+    def getHusimiQGrid(self, regIdx: int, qs, ps, per_slice: bool = False) -> npt.NDArray[np.float64]:
+        """Husimi-Q of register regIdx at the points (qs, ps): shape (len(ps), len(qs)), or
+        (S, len(ps), len(qs)) per slice; equals getHusimiQ() on its grid."""
+        return self._grid("cvdvGetHusimiQGrid", regIdx, qs, ps, per_slice)
+
     def synchronize(self) -> None:
         self._c("cvdvSynchronize")
 

@@ -106,6 +106,22 @@ void   cvdvProfileEnable(CVDVContext* ctx, int on);
 int    cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs);
 void   cvdvProfileReset(CVDVContext* ctx);
 
+/* ---- phase space on arbitrary points (addition) ----
+ * Wigner / Husimi-Q of register regIdx at the points (xs[i], ps[k]) (host
+ * arrays), written row-major out[k * nx + i] (rows momentum, columns position,
+ * as cvdvGetWigner).  perSlice = 0: traced over the other registers;
+ * perSlice = 1: one np x nx block per slice, slices in the canonical order of
+ * the other registers' flat index (one block per branch).  The Wigner is the
+ * discrete sum of cvdvGetWigner with each slice's trigonometric interpolant
+ * between grid points; native = 1 keeps cvdvGetWigner's e^{-i (N-1) dx p}
+ * finalisation (equal to cvdvGetWigner on its grid), native = 0 drops it.
+ * The Husimi is cvdvGetHusimiQ's overlap sum at arbitrary (q, p) (equal to it
+ * on its grid).  FP64 throughout. */
+void   cvdvGetWignerGrid(CVDVContext* ctx, int regIdx, const double* xs, int nx, const double* ps, int np,
+                         int perSlice, int native, double* out);
+void   cvdvGetHusimiQGrid(CVDVContext* ctx, int regIdx, const double* qs, int nq, const double* ps, int np,
+                          int perSlice, double* out);
+
 /* ---- batched circuits (addition) ----
  * Many small independent circuits executed by one kernel launch, one
  * workgroup per circuit.  numQubitsFlat lists every circuit's register sizes

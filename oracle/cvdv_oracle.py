@@ -58,6 +58,22 @@ def _floor_half_up(x: float) -> int:
     return int(np.floor(x + 0.5))
 
 
+def _grid_split(u: float, dim: int):
+    """u = position in grid units from x_0 -> (i, t), u = i + t, t in [0, 1); points
+    within 1e-9 of a grid point snap to it (t = 0) so native points are exact."""
+    r = round(u)
+    if abs(u - r) < 1e-9:
+        i, t = int(r), 0.0
+    else:
+        i = int(np.floor(u))
+        t = u - i
+    if i < 0:
+        i, t = 0, 0.0
+    if i > dim - 1:
+        i, t = dim - 1, 0.0
+    return i, t
+
+
 class CvdvOracle:
     """FP64 dense reference simulator (restates torchCvdv.py:14-528)."""
 
@@ -523,6 +539,58 @@ class CvdvOracle:
         xv = (np.arange(dim) - (dim - 1) / 2) * dx
         pv = (np.arange(dim) - dim / 2) * dp
         return W[np.ix_(np.abs(pv) <= bound, np.abs(xv) <= bound)]
+
+    def getWignerGrid(self, regIdx: int, xs, ps, per_slice: bool = False,
+                      native: bool = True) -> np.ndarray:
+        """Extension (no reference counterpart): the discrete Wigner sum of getWigner
+        (torchCvdv.py:344-374) at arbitrary points.  With x = x_i + t dx (t in [0, 1),
+        t = 0 on grid points) and A(d) = sum_s phi_s(i-d) conj(phi_s(i+d)), phi_s the
+        trigonometric interpolant of slice s sampled at x_j + t dx:
+            W(x, p) = dx/pi Re[e^{-i (N-1) dx p} sum_d A(d) e^{2 i p dx (d + (N-1)//2)}]
+        (native=True; on the native grid this is getWigner exactly), or without the
+        constant-offset phase (native=False): dx/pi Re[sum_d A(d) e^{2 i p dx d}].
+        Output (len(ps), len(xs)); per_slice: (S, len(ps), len(xs))."""
+        dim = self.register_dims[regIdx]
+        dx = self.grid_steps[regIdx]
+        sl = self._slices(regIdx)
+        S = sl.shape[0]
+        xs = np.asarray(xs, dtype=np.float64)
+        ps = np.asarray(ps, dtype=np.float64)
+        c = np.fft.fft(sl, axis=1)
+        kk = np.fft.fftfreq(dim) * dim
+        x0 = -(dim - 1) / 2 * dx
+        o = (dim - 1) // 2
+        out = np.zeros((S if per_slice else 1, len(ps), len(xs)))
+        for ix, x in enumerate(xs):
+            i, t = _grid_split((x - x0) / dx, dim)
+            phi = sl if t == 0.0 else np.fft.ifft(c * np.exp(2j * PI * kk * t / dim), axis=1)
+            dmax = min(i, dim - 1 - i)
+            d = np.arange(-dmax, dmax + 1)
+            A = phi[:, i - d] * np.conj(phi[:, i + d])  # (S, nd)
+            if not per_slice:
+                A = A.sum(axis=0, keepdims=True)
+            if native:
+                E = np.exp(1j * dx * (np.outer(ps, 2 * (d + o)) - (dim - 1) * ps[:, None]))
+            else:
+                E = np.exp(2j * dx * np.outer(ps, d))
+            out[:, :, ix] = (A @ E.T).real * (dx / PI)
+        return out if per_slice else out[0]
+
+    def getHusimiQGrid(self, regIdx: int, qs, ps, per_slice: bool = False) -> np.ndarray:
+        """Extension (no reference counterpart): getHusimiQ's sum (torchCvdv.py:376-404)
+        at arbitrary points, Q(q, p) = dx/pi^(3/2) sum_s |sum_j psi_s(x_j)
+        e^{-(x_j-q)^2/2} e^{-i p x_j}|^2 (on the native grid this is getHusimiQ
+        exactly).  Output (len(ps), len(qs)); per_slice: (S, len(ps), len(qs))."""
+        dx = self.grid_steps[regIdx]
+        sl = self._slices(regIdx)
+        xg = self._x(regIdx)
+        qs = np.asarray(qs, dtype=np.float64)
+        ps = np.asarray(ps, dtype=np.float64)
+        B = sl[:, :, None] * np.exp(-0.5 * (xg[:, None] - qs[None, :]) ** 2)[None]  # (S, N, nq)
+        E = np.exp(-1j * np.outer(ps, xg))  # (np, N)
+        H = np.einsum("pj,sjq->spq", E, B)
+        Q = np.abs(H) ** 2 * (dx / PI ** 1.5)
+        return Q if per_slice else Q.sum(axis=0)
 
     def getHusimiQ(self, regIdx: int, bound: float | None = None) -> np.ndarray:
         # torchCvdv.py:376-404 (non-periodic Gaussian window)

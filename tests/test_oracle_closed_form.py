@@ -235,3 +235,61 @@ def test_conventions_differ_only_where_documented():
     assert abs(s.m() - 4.0) < 1e-14
     t, _ = _sim([4], [["setCoeffs", 0, np.full(16, 0.5)]], conv="torch")
     assert abs(t.m() - 1.0) < 1e-14
+
+
+# ---------------------------------------------------------------- grid readouts (extension)
+def _grid_state():
+    s, _ = _sim([2, 8], [["setUniform", 0], ["setCoherent", 1, 0.9 - 0.4j]], conv="cuda")
+    s.cd(1, 0, 0, 0.6 + 0.2j)
+    s.cr(1, 0, 1, 0.4)
+    return s
+
+
+def test_grid_wigner_husimi_equal_native_on_native_points():
+    """On its own grid the extension is the native readout exactly (both registers'
+    slices, traced)."""
+    s = _grid_state()
+    N, dx = 256, s.grid_steps[1]
+    xv = (np.arange(N) - (N - 1) / 2) * dx
+    ix, ip = np.arange(1, N, 9), np.arange(4, N, 11)
+    W = s.getWigner(1)
+    pw = (np.arange(N) - N / 2) * pi / (N * dx)
+    np.testing.assert_allclose(s.getWignerGrid(1, xv[ix], pw[ip]), W[np.ix_(ip, ix)], atol=1e-14)
+    Q = s.getHusimiQ(1)
+    ph = (np.arange(N) - N / 2) * 2 * pi / (N * dx)
+    np.testing.assert_allclose(s.getHusimiQGrid(1, xv[ix], ph[ip]), Q[np.ix_(ip, ix)], atol=1e-14)
+
+
+def test_grid_readouts_coherent_closed_form_off_grid():
+    """Off-grid points of a coherent state: dx/pi e^{-(x-q0)^2-(p-p0)^2} (textbook sum),
+    the same times cos(p dx) with the native offset phase, and the Husimi
+    e^{-((q-q0)^2+(p-p0)^2)/2}/pi; the grid spacing (0.23) is not a multiple of dx."""
+    a = 1.1 - 0.6j
+    q0, p0 = sqrt(2) * a.real, sqrt(2) * a.imag
+    for n in (8, 10):
+        s, _ = _sim([n], [["setCoherent", 0, a]], conv="cuda")
+        dx = s.grid_steps[0]
+        xs, ps = np.linspace(-4, 4.2, 37), np.linspace(-3.9, 4, 31)
+        We = dx / pi * np.exp(-(xs[None, :] - q0) ** 2 - (ps[:, None] - p0) ** 2)
+        assert np.max(np.abs(s.getWignerGrid(0, xs, ps, native=False) - We)) < 1e-12
+        assert np.max(np.abs(s.getWignerGrid(0, xs, ps) - We * np.cos(ps[:, None] * dx))) < 1e-12
+        Qe = np.exp(-0.5 * (xs[None, :] - q0) ** 2 - 0.5 * (ps[:, None] - p0) ** 2) / pi
+        assert np.max(np.abs(s.getHusimiQGrid(0, xs, ps) - Qe)) < 1e-12
+
+
+def test_grid_readouts_per_slice_linearity_and_branches():
+    """Per-slice blocks sum to the traced readout; each slice is the readout of that
+    branch alone (the state projected on the other registers' values)."""
+    s = _grid_state()
+    xs, ps = np.linspace(-3, 3, 13), np.linspace(-2.5, 2.5, 11)
+    Ws = s.getWignerGrid(1, xs, ps, per_slice=True)
+    Qs = s.getHusimiQGrid(1, xs, ps, per_slice=True)
+    assert Ws.shape == (4, 11, 13) and Qs.shape == (4, 11, 13)
+    np.testing.assert_allclose(Ws.sum(0), s.getWignerGrid(1, xs, ps), atol=1e-15)
+    np.testing.assert_allclose(Qs.sum(0), s.getHusimiQGrid(1, xs, ps), atol=1e-15)
+    assert np.all(Qs >= 0)
+    for b in range(4):
+        one = CvdvOracle([8], convention="cuda")
+        one.initFromArray(s.state[b])
+        np.testing.assert_allclose(Ws[b], one.getWignerGrid(0, xs, ps), atol=1e-15)
+        np.testing.assert_allclose(Qs[b], one.getHusimiQGrid(0, xs, ps), atol=1e-15)

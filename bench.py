@@ -253,6 +253,79 @@ def c4_section(n, n_seq, steps):
     }
 
 
+# ---------------------------------------------------------------- state transfer (C3)
+def make_c3(nq=14, ncv=14, seed=3):
+    """Oscillator-to-qubit transfer: registers [1]*nq + [ncv] (oscillator last, its axis
+    contiguous); the oscillator starts in a normalised superposition of 3 coherent
+    states (|alpha| ~ U[0,3], arg ~ U[0,2pi), complex-normal coefficients), the qubits in
+    |0>; round k = cd(osc, k, 0, beta_k), h(k, 0), cr(osc, k, 0, theta_k) with beta_k in
+    the unit disc and theta_k ~ U(-pi, pi)."""
+    rng = np.random.default_rng(seed)
+    cat = []
+    for _ in range(3):
+        rad, ph = 3 * rng.random(), rng.uniform(0, 2 * math.pi)
+        cat.append((complex(rad * math.cos(ph), rad * math.sin(ph)), complex(*rng.normal(size=2))))
+    ops = []
+    osc = nq
+    for k in range(nq):
+        rad, ph = math.sqrt(rng.random()), rng.uniform(0, 2 * math.pi)
+        ops.append(("cd", osc, k, 0, complex(rad * math.cos(ph), rad * math.sin(ph))))
+        ops.append(("h", k, 0))
+        ops.append(("cr", osc, k, 0, float(rng.uniform(-math.pi, math.pi))))
+    return [1] * nq + [ncv], cat, ops
+
+
+def c3_section(steps):
+    import torch
+    from hipcvdv import HipCvdv, SeparableState
+
+    regs, cat, ops = make_c3()
+    nq = len(regs) - 1
+    res = {}
+    joint = {}
+    for prec in ("mixed", "complex128"):
+        h = HipCvdv(regs, precision=prec)
+        sep = SeparableState(regs)
+        for k in range(nq):
+            sep.setZero(k)
+        sep.setCat(nq, cat)
+
+        def run():
+            h.initStateVector(sep)
+            for op in ops:
+                getattr(h, op[0])(*op[1:])
+            h.synchronize()
+
+        run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            run()
+        t_circ = (time.perf_counter() - t0) / steps
+        t0 = time.perf_counter()
+        singles = [h.m(k) for k in range(nq)]
+        t_single = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        jp = h.m(*range(nq))
+        t_joint = time.perf_counter() - t0
+        joint[prec] = (np.asarray(jp), [np.asarray(x) for x in singles])
+        res[prec] = {"circuit_ms": t_circ * 1e3, "single_marginals_ms": t_single * 1e3,
+                     "joint_marginal_ms": t_joint * 1e3, "joint_bins": int(np.asarray(jp).size),
+                     "prob_sum": float(np.asarray(jp).sum())}
+        print(f"[bench] c3 {prec}: circuit {t_circ * 1e3:.1f} ms, 14 marginals {t_single * 1e3:.1f} ms, "
+              f"joint {t_joint * 1e3:.1f} ms", file=sys.stderr, flush=True)
+        del h
+        torch.cuda.empty_cache()
+    jm, sm = joint["mixed"]
+    jd, sd = joint["complex128"]
+    return {"workload": "[1]x14 + [14] state transfer, 3-component cat, 14 rounds cd/h/cr, seed 3",
+            "mixed": res["mixed"], "complex128": res["complex128"],
+            "joint_max_abs_diff_mixed_vs_fp64": float(np.max(np.abs(jm - jd))),
+            "single_max_abs_diff_mixed_vs_fp64": float(max(np.max(np.abs(a - b)) for a, b in zip(sm, sd))),
+            "single_vs_joint_consistency_fp64": float(max(
+                np.max(np.abs(sd[k] - jd.sum(axis=tuple(i for i in range(nq) if i != k)))) for k in range(nq)))}
+
+
 def run_gpu(prec, ops, steps, warmup):
     import torch
     from hipcvdv import HipCvdv, SeparableState
@@ -358,6 +431,7 @@ def main():
     ap.add_argument("--c4-circuits", type=int, default=C4_N)
     ap.add_argument("--c4-seq", type=int, default=64, help="circuits timed one by one")
     ap.add_argument("--c4-only", action="store_true", help="run only the batched-circuit workload")
+    ap.add_argument("--no-c3", action="store_true", help="skip the state-transfer workload")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the state vector is not sharded across GPUs.",
@@ -425,6 +499,8 @@ def main():
         out["max_rel_amp_err_mixed_vs_fp64"] = rel_err
         out["norm_fp64"] = n0
         out["norm_mixed"] = n1
+    if not args.no_c3 and not args.only:
+        out["c3"] = c3_section(max(1, args.steps))
     if not args.no_c4 and not args.only:
         out["c4"] = c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))
     if not args.no_cpu_baseline:

@@ -7,10 +7,12 @@
 
 #include <cmath>
 #include <cstdio>
+#include <algorithm>
 #include <cstring>
 #include <new>
 
 #include "cvdv_internal.h"
+#include "diag_fast.h"
 #include "../../include/cvdv_hip.h"
 
 // ---------------------------------------------------------------- errors
@@ -48,7 +50,10 @@ void cvdv_check_hip(hipError_t e, const char* what) {
 // ---------------------------------------------------------------- context helpers
 void* CVDVContext::other() {
     const int o = 1 - cur;
-    if (!buf[o]) HIPCHK(hipMalloc(&buf[o], totalSize * elemBytes()));
+    if (!buf[o]) {
+        if (capturing) throw CvdvError{CVDV_ERR_UNSUPPORTED, "buffer allocation inside a graph capture"};
+        HIPCHK(hipMalloc(&buf[o], totalSize * elemBytes()));
+    }
     return buf[o];
 }
 
@@ -60,6 +65,7 @@ bool CVDVContext::canonical() const {
 
 void* ctx_scratch(CVDVContext* ctx, size_t bytes) {
     if (bytes > ctx->scratchBytes) {
+        if (ctx->capturing) throw CvdvError{CVDV_ERR_UNSUPPORTED, "scratch allocation inside a graph capture"};
         if (ctx->scratch) HIPCHK(hipFree(ctx->scratch));
         ctx->scratch = nullptr;
         ctx->scratchBytes = 0;
@@ -82,7 +88,7 @@ static hipEvent_t take_event(CVDVContext* ctx) {
 }
 
 int prof_begin(CVDVContext* ctx, int kind) {
-    if (!ctx->profOn) return -1;
+    if (!ctx->profOn || ctx->capturing) return -1;
     ProfRec r{kind, take_event(ctx), take_event(ctx)};
     HIPCHK(hipEventRecord(r.a, ctx->stream));
     ctx->prof.push_back(r);
@@ -116,9 +122,12 @@ static void ctx_release(CVDVContext* ctx) {
     if (ctx->arena) (void)hipFree(ctx->arena);
     ctx->arena = nullptr;
     ctx->arenaBytes = 0;
+    for (void* p : ctx->retired) (void)hipFree(p);
+    ctx->retired.clear();
     if (ctx->tw) twiddle_cache_free(ctx->tw);
     ctx->tw = nullptr;
     blas_release(ctx);
+    ctx->generation++;  // captured graphs of the released buffers become invalid
     ctx->initialized = false;
 }
 
@@ -210,6 +219,13 @@ __global__ void k_to_c128(double2* __restrict__ out, const C* __restrict__ in, u
         out[g] = make_double2((double)in[g].x, (double)in[g].y);
 }
 
+template <typename C>
+__global__ void k_copy(C* __restrict__ out, const C* __restrict__ in, uint64_t total) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
+         g += (uint64_t)gridDim.x * blockDim.x)
+        out[g] = in[g];
+}
+
 static unsigned grid_for(uint64_t total, unsigned block = 256) {
     uint64_t b = (total + block - 1) / block;
     if (b > 65536ull * 8) b = 65536ull * 8;
@@ -246,6 +262,40 @@ void ctx_materialize(CVDVContext* ctx) {
         ctx->pshift[r] = ctx->flw[r];
         ctx->refl[r] = 0;
     }
+}
+
+// ---------------------------------------------------------------- graph capture
+// A captured gate sequence.  Capture starts from the canonical layout with no
+// pending phases in buffer `cur`, and ends the same way (pending phases
+// flushed, layout materialised, the result copied back to `cur` if the last
+// pass left it in the other buffer), so a replay maps that state to the state
+// after the sequence and replays compose.
+struct CVDVGraph {
+    CVDVContext* ctx;
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int cur;
+    int generation;
+};
+
+struct CaptureSave {
+    int pshift[CVDV_MAX_REG], refl[CVDV_MAX_REG];
+    DiagProg pending;
+    int cur;
+};
+static thread_local CaptureSave g_capSave;
+
+// Copy the state between the two buffers (a kernel, so it can sit in a graph).
+static void copy_to(CVDVContext* ctx, int dst) {
+    const uint64_t n = ctx->totalSize;
+    if (ctx->prec == CVDV_C128)
+        hipLaunchKernelGGL(k_copy<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (double2*)ctx->buf[dst], (const double2*)ctx->buf[ctx->cur], n);
+    else
+        hipLaunchKernelGGL(k_copy<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+                           (float2*)ctx->buf[dst], (const float2*)ctx->buf[ctx->cur], n);
+    HIPCHK(hipGetLastError());
+    ctx->cur = dst;
 }
 
 // ---------------------------------------------------------------- C ABI
@@ -345,6 +395,10 @@ int cvdvGetLastError(char* buf, size_t len) {
 }
 
 void cvdvSynchronize(CVDVContext* ctx) {
+    if (ctx && ctx->capturing) {
+        cvdv_set_error(CVDV_ERR_STATE, "the context is capturing a graph: end the capture first");
+        return;
+    }
     CVDV_TRY
     if (ctx && ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
     CVDV_CATCH()
@@ -507,6 +561,97 @@ void cvdvGetState(CVDVContext* ctx, double* realOut, double* imagOut) {
     HIPCHK(hipMemcpyAsync(imagOut, d + n, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
     CVDV_CATCH()
+}
+
+void cvdvGraphBegin(CVDVContext* ctx) {
+    if (!ctx || refuse_recording(ctx)) return;
+    if (!ctx->initialized) {
+        cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
+        return;
+    }
+    CVDV_TRY
+    ctx_materialize(ctx);
+    (void)ctx->other();  // both buffers exist before the capture
+    int maxNb = 0;
+    for (int r = 0; r < ctx->numReg; ++r) maxNb = std::max(maxNb, ctx->nq[r]);
+    arena_reserve(ctx, ((size_t)CVDV_FAST_TABS * ((size_t)1 << std::min(maxNb, 20)) + 1) * ctx->elemBytes());
+    fft_prepare(ctx);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < ctx->numReg; ++r) {
+        g_capSave.pshift[r] = ctx->pshift[r];
+        g_capSave.refl[r] = ctx->refl[r];
+    }
+    g_capSave.pending = ctx->pending;
+    g_capSave.cur = ctx->cur;
+    HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    CVDV_CATCH()
+}
+
+CVDVGraph* cvdvGraphEnd(CVDVContext* ctx) {
+    if (!ctx) return nullptr;
+    if (!ctx->capturing) {
+        cvdv_set_error(CVDV_ERR_STATE, "cvdvGraphEnd: no capture in progress");
+        return nullptr;
+    }
+    hipGraph_t g = nullptr;
+    bool ok = true;
+    std::string why;
+    try {
+        ctx_materialize(ctx);
+        if (ctx->cur != g_capSave.cur) copy_to(ctx, g_capSave.cur);
+    } catch (const CvdvError& e) {
+        ok = false;
+        why = e.msg;
+    }
+    ctx->capturing = false;
+    const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
+    // the host-side layout returns to the capture start: nothing has executed
+    for (int r = 0; r < ctx->numReg; ++r) {
+        ctx->pshift[r] = g_capSave.pshift[r];
+        ctx->refl[r] = g_capSave.refl[r];
+    }
+    ctx->pending = g_capSave.pending;
+    ctx->cur = g_capSave.cur;
+    if (!ok || ce != hipSuccess || !g) {
+        if (g) (void)hipGraphDestroy(g);
+        cvdv_set_error(CVDV_ERR_HIP, "cvdvGraphEnd: capture failed" + (why.empty() ? std::string() : ": " + why));
+        return nullptr;
+    }
+    hipGraphExec_t e = nullptr;
+    if (hipGraphInstantiate(&e, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        cvdv_set_error(CVDV_ERR_HIP, "cvdvGraphEnd: hipGraphInstantiate failed");
+        return nullptr;
+    }
+    ctx->graphs++;
+    return new CVDVGraph{ctx, g, e, g_capSave.cur, ctx->generation};
+}
+
+void cvdvGraphLaunch(CVDVGraph* gr) {
+    if (!gr) return;
+    CVDVContext* ctx = gr->ctx;
+    if (ctx->capturing || refuse_recording(ctx)) {
+        cvdv_set_error(CVDV_ERR_STATE, "cvdvGraphLaunch: the context is capturing");
+        return;
+    }
+    if (!ctx->initialized || gr->generation != ctx->generation) {
+        cvdv_set_error(CVDV_ERR_STATE, "cvdvGraphLaunch: the context's state was released after the capture");
+        return;
+    }
+    CVDV_TRY
+    ctx_materialize(ctx);
+    if (ctx->cur != gr->cur) copy_to(ctx, gr->cur);
+    HIPCHK(hipGraphLaunch(gr->exec, ctx->stream));
+    CVDV_CATCH()
+}
+
+void cvdvGraphDestroy(CVDVGraph* gr) {
+    if (!gr) return;
+    if (gr->ctx && gr->ctx->stream) (void)hipStreamSynchronize(gr->ctx->stream);
+    if (gr->exec) (void)hipGraphExecDestroy(gr->exec);
+    if (gr->graph) (void)hipGraphDestroy(gr->graph);
+    delete gr;
 }
 
 void cvdvGetStateDevice(CVDVContext* ctx, void* d_dst) {

@@ -194,6 +194,10 @@ struct CVDVContext {
     size_t arenaBytes = 0;
     BatchRecording* rec = nullptr;    // non-null: record ops, launch nothing
     void* blas = nullptr;             // rocBLAS handle of the grid readouts (created on first use)
+    bool capturing = false;           // between cvdvGraphBegin and cvdvGraphEnd
+    int graphs = 0;                   // graphs captured (they hold arena pointers)
+    std::vector<void*> retired;       // arenas outgrown while a graph may still read them
+    int generation = 0;               // bumped when the state buffers are released
     bool profOn = false;
     std::vector<ProfRec> prof;        // recorded launches
     std::vector<hipEvent_t> evPool;   // reusable events
@@ -241,9 +245,13 @@ void* ctx_scratch(CVDVContext* ctx, size_t bytes);
 
 // batch recording (gates.hip): append an op whose phase program is `p`
 void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p);
+// Everything a gate may allocate or upload, done before a graph capture (fft.hip)
+void fft_prepare(CVDVContext* ctx);
+void arena_reserve(CVDVContext* ctx, size_t bytes);  // gates.hip
 // rocBLAS handle of the grid readouts (phase_grid.hip)
 void blas_release(CVDVContext* ctx);
-// readouts refuse recording contexts (no state of their own)
+// readouts refuse recording contexts (no state of their own) and contexts that
+// are capturing a graph (nothing has executed yet)
 bool refuse_recording(CVDVContext* ctx);
 // Stockham stage twiddles of length 2^logM in the given precision (fft.hip):
 // device table (cached in tc) and a host copy for tables built elsewhere.

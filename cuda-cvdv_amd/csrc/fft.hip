@@ -188,6 +188,8 @@ const void* fft_stage_table(TwiddleCache*& tcp, int logM, int prec) {
 }
 
 static void* stage_table(CVDVContext* ctx, int logM, int prec) {
+    if (ctx->capturing && (!ctx->tw || !ctx->tw->stage.count({logM, prec})))
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT table not prepared before the graph capture"};
     return (void*)fft_stage_table(ctx->tw, logM, prec);
 }
 
@@ -197,6 +199,7 @@ static const void* full_table(CVDVContext* ctx, int logN, int prec = CVDV_C128) 
     TwiddleCache* tc = cache(ctx);
     auto it = tc->full.find({logN, prec});
     if (it != tc->full.end()) return it->second;
+    if (ctx->capturing) throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT table not prepared before the graph capture"};
     const uint64_t N = 1ull << logN;
     std::vector<double> re(N), im(N);
     for (uint64_t e = 0; e < N; ++e) host_twiddle(e, N, re[e], im[e]);
@@ -691,6 +694,17 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     launch_pass<R>(ctx, data, data, d1, pre, prec, H);
     launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, prec, H);
     return true;
+}
+
+// Every twiddle table an FFT of any register of the context can ask for.
+void fft_prepare(CVDVContext* ctx) {
+    for (int r = 0; r < ctx->numReg; ++r) {
+        const int nb = ctx->nq[r];
+        if (nb == 0 || nb > 28) continue;  // above 28 qubits the two-pass FFT refuses anyway
+        for (int lg = 1; lg <= nb && lg <= 14; ++lg) (void)stage_table(ctx, lg, ctx->prec);
+        (void)full_table(ctx, nb, ctx->prec);                   // four-step twiddles
+        if (nb <= 12) (void)full_table(ctx, nb, CVDV_C128);     // direct DFT of tiny states
+    }
 }
 
 void fft_register(CVDVContext* ctx, int pshift, int nbits, bool inverse, const DiagProg& pre,

@@ -27,17 +27,13 @@ __global__ void k_diag(C* __restrict__ s, uint64_t total, FastDiag f) {
 }
 
 // A whole program evaluated per element in FP64 (programs beyond the compiled
-// capacities; rare).  The program sits in device memory: a kernel-argument
-// copy indexed by a runtime term count would be spilled to scratch.
+// capacities; rare).
 template <typename C>
-__global__ void k_diag_prog(C* __restrict__ s, uint64_t total, const DiagProg* __restrict__ p) {
+__global__ void k_diag_prog(C* __restrict__ s, uint64_t total, DiagProg p) {
     using R = decltype(C{}.x);
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        double r = p->grev;
-        for (int i = 0; i < p->n; ++i) r += cvdv_frac(diag_term_rev(p->t[i], g));
-        s[g] = cmul(s[g], unit_rev(cvdv_frac(r), R(0)));
-    }
+         g += (uint64_t)gridDim.x * blockDim.x)
+        s[g] = cmul(s[g], unit_rev(diag_rev(p, g), R(0)));
 }
 
 struct TabMake {
@@ -152,21 +148,33 @@ static bool term_is_identity(const DiagTerm& t) {
 // The whole program as one FP64 pass over the state (programs that exceed the
 // compiled capacities).
 static void diag_prog_pass(CVDVContext* ctx, const DiagProg& p) {
-    const size_t need = sizeof(DiagProg);
-    DiagProg* d = (DiagProg*)ctx_scratch(ctx, need);
-    HIPCHK(hipMemcpyAsync(d, &p, need, hipMemcpyHostToDevice, ctx->stream));
     const uint64_t n = ctx->totalSize;
     const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_diag_prog<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                           (double2*)ctx->state(), n, (const DiagProg*)d);
+                           (double2*)ctx->state(), n, p);
     else
         hipLaunchKernelGGL(k_diag_prog<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                           (float2*)ctx->state(), n, (const DiagProg*)d);
+                           (float2*)ctx->state(), n, p);
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
-    // the host copy must outlive the asynchronous transfer
+}
+
+// Grow the phase-table arena (synchronises: earlier launches may still read it).
+void arena_reserve(CVDVContext* ctx, size_t need) {
+    if (need <= ctx->arenaBytes) return;
+    if (ctx->capturing)
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "phase tables outgrew the arena reserved for the graph capture"};
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    if (ctx->arena) {
+        if (ctx->graphs > 0) ctx->retired.push_back(ctx->arena);  // a captured graph may replay into it
+        else HIPCHK(hipFree(ctx->arena));
+    }
+    ctx->arena = nullptr;
+    ctx->arenaBytes = 0;
+    const size_t sz = need < (size_t)(1 << 20) ? (size_t)(1 << 20) : need;
+    HIPCHK(hipMalloc(&ctx->arena, sz));
+    ctx->arenaBytes = sz;
 }
 
 FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale, bool* scaleFolded) {
@@ -230,16 +238,7 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double 
     active = f.ntab > 0 || f.ngen > 0;
     if (a.n) {
         a.total = off;
-        const size_t need = (size_t)off * ctx->elemBytes();
-        if (need > ctx->arenaBytes) {
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            if (ctx->arena) HIPCHK(hipFree(ctx->arena));
-            ctx->arena = nullptr;
-            ctx->arenaBytes = 0;
-            const size_t sz = need < (size_t)(1 << 20) ? (size_t)(1 << 20) : need;
-            HIPCHK(hipMalloc(&ctx->arena, sz));
-            ctx->arenaBytes = sz;
-        }
+        arena_reserve(ctx, (size_t)off * ctx->elemBytes());
         const unsigned blocks = (unsigned)((off + 255) / 256);
         if (ctx->prec == CVDV_C128)
             hipLaunchKernelGGL(k_make_tables<double2>, dim3(blocks), dim3(256), 0, ctx->stream,
@@ -295,6 +294,10 @@ void rec_op(CVDVContext* ctx, BatchOp op, const DiagProg& p) {
 bool refuse_recording(CVDVContext* ctx) {
     if (ctx && ctx->rec) {
         cvdv_set_error(CVDV_ERR_STATE, "batch circuit context: run the batch and read it through the batch API");
+        return true;
+    }
+    if (ctx && ctx->capturing) {
+        cvdv_set_error(CVDV_ERR_STATE, "the context is capturing a graph: end the capture first");
         return true;
     }
     return false;

@@ -68,6 +68,10 @@ SIGNATURES = {
     "cvdvProfileEnable": ([c_void_p, c_int], None),
     "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),
     "cvdvProfileReset": ([c_void_p], None),
+    "cvdvGraphBegin": ([c_void_p], None),
+    "cvdvGraphEnd": ([c_void_p], c_void_p),
+    "cvdvGraphLaunch": ([c_void_p], None),
+    "cvdvGraphDestroy": ([c_void_p], None),
     "cvdvGetWignerGrid": ([c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_double), c_int, c_int, c_int,
                            POINTER(c_double)], None),
     "cvdvGetHusimiQGrid": ([c_void_p, c_int, POINTER(c_double), c_int, POINTER(c_double), c_int, c_int,

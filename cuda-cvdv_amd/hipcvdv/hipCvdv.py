@@ -290,6 +290,11 @@ class HipCvdv:
         (S, len(ps), len(qs)) per slice; equals getHusimiQ() on its grid."""
         return self._grid("cvdvGetHusimiQGrid", regIdx, qs, ps, per_slice)
 
+    def capture(self) -> "HipCvdvGraph":
+        """Record the gate calls of a ``with`` block as a hipGraph instead of running
+        them; ``launch()`` the result to apply them (replays compose)."""
+        return HipCvdvGraph(self)
+
     def synchronize(self) -> None:
         self._c("cvdvSynchronize")
 
@@ -342,3 +347,40 @@ class HipCvdv:
         if show:
             plt.show()
         return fig, ax
+
+
+class HipCvdvGraph:
+    """A gate sequence captured from a HipCvdv (see HipCvdv.capture)."""
+
+    def __init__(self, sim: HipCvdv) -> None:
+        self.sim = sim
+        self.handle = None
+
+    def __enter__(self) -> "HipCvdvGraph":
+        self.sim._c("cvdvGraphBegin")
+        return self
+
+    def __exit__(self, exc_type, exc, tb) -> bool:
+        handle = self.sim._lib.cvdvGraphEnd(self.sim.ctx)
+        if exc_type is not None:
+            if handle:
+                self.sim._lib.cvdvGraphDestroy(handle)
+            _lib.get_lib().cvdvGetLastError(None, 0)
+            return False
+        _lib.check()
+        self.handle = handle
+        return False
+
+    def launch(self) -> None:
+        if not self.handle:
+            raise RuntimeError("no captured graph")
+        self.sim._lib.cvdvGraphLaunch(self.handle)
+        _lib.check()
+
+    def __del__(self):
+        try:
+            if self.handle:
+                self.sim._lib.cvdvGraphDestroy(self.handle)
+                self.handle = None
+        except Exception:
+            pass

@@ -106,6 +106,21 @@ void   cvdvProfileEnable(CVDVContext* ctx, int on);
 int    cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs);
 void   cvdvProfileReset(CVDVContext* ctx);
 
+/* ---- graph capture of a gate sequence (addition) ----
+ * cvdvGraphBegin brings the context to its canonical layout with no pending
+ * phases, allocates everything a gate may need and starts capturing the
+ * context's stream; the gate calls that follow are recorded, not executed, and
+ * readouts are refused until cvdvGraphEnd, which returns the sequence as an
+ * instantiated hipGraph (NULL on failure; the state is untouched either way).
+ * cvdvGraphLaunch applies the sequence to the context's current state
+ * (asynchronously, replays compose).  Destroy graphs before their context; a
+ * graph is refused after cvdvFree of its context. */
+typedef struct CVDVGraph CVDVGraph;
+void   cvdvGraphBegin(CVDVContext* ctx);
+CVDVGraph* cvdvGraphEnd(CVDVContext* ctx);
+void   cvdvGraphLaunch(CVDVGraph* graph);
+void   cvdvGraphDestroy(CVDVGraph* graph);
+
 /* ---- phase space on arbitrary points (addition) ----
  * Wigner / Husimi-Q of register regIdx at the points (xs[i], ps[k]) (host
  * arrays), written row-major out[k * nx + i] (rows momentum, columns position,

@@ -326,6 +326,113 @@ def c3_section(steps):
                 np.max(np.abs(sd[k] - jd.sum(axis=tuple(i for i in range(nq) if i != k)))) for k in range(nq)))}
 
 
+# ---------------------------------------------------------------- phase-space readouts (C1, C5)
+def make_c5(seed=5, cv=12, ngates=20):
+    """[2,2,cv]: two 2-qubit registers in uniform superposition, an oscillator in a coherent
+    state (seeded, |Re|,|Im| <= 1); ngates seeded gates, each a conditional displacement
+    (|Re|,|Im| <= 0.8) or conditional rotation (theta ~ U(-pi, pi)) of the oscillator
+    controlled by one of the 4 qubits."""
+    rng = np.random.default_rng(seed)
+    alpha = complex(*rng.uniform(-1, 1, 2))
+    ops = []
+    for _ in range(ngates):
+        creg = int(rng.integers(0, 2))
+        cq = int(rng.integers(0, 2))
+        if rng.random() < 0.5:
+            ops.append(("cd", 2, creg, cq, complex(*rng.uniform(-0.8, 0.8, 2))))
+        else:
+            ops.append(("cr", 2, creg, cq, float(rng.uniform(-math.pi, math.pi))))
+    return [2, 2, cv], alpha, ops
+
+
+def c5_section(steps):
+    import torch
+    from hipcvdv import HipCvdv, SeparableState
+
+    regs, alpha, ops = make_c5()
+    grid = np.linspace(-6, 6, 512)
+    res, outs = {}, {}
+    for prec in ("mixed", "complex128"):
+        h = HipCvdv(regs, precision=prec)
+        sep = SeparableState(regs)
+        sep.setUniform(0)
+        sep.setUniform(1)
+        sep.setCoherent(2, alpha)
+        h.initStateVector(sep)
+        for op in ops:
+            getattr(h, op[0])(*op[1:])
+        h.synchronize()
+        h.getWignerGrid(2, grid[:8], grid[:8], per_slice=True)  # warm (library load)
+        t = {}
+        for name, fn in (("wigner_512_per_branch", lambda: h.getWignerGrid(2, grid, grid, per_slice=True)),
+                         ("husimi_512_per_branch", lambda: h.getHusimiQGrid(2, grid, grid, per_slice=True)),
+                         ("wigner_native_traced", lambda: h.getWigner(2)),
+                         ("husimi_native_traced", lambda: h.getHusimiQ(2)),
+                         ("joint_m012", lambda: h.m(0, 1, 2))):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                val = fn()
+            t[name + "_ms"] = (time.perf_counter() - t0) / steps * 1e3
+            outs[(prec, name)] = np.asarray(val)
+        res[prec] = t
+        print(f"[bench] c5 {prec}: " + ", ".join(f"{k} {v:.1f}" for k, v in t.items()), file=sys.stderr, flush=True)
+        del h
+        torch.cuda.empty_cache()
+    err = {}
+    for name in ("wigner_512_per_branch", "husimi_512_per_branch", "wigner_native_traced", "husimi_native_traced",
+                 "joint_m012"):
+        a, b = outs[("mixed", name)], outs[("complex128", name)]
+        err[name] = {"max_abs_err": float(np.max(np.abs(a - b))), "max_abs_fp64": float(np.max(np.abs(b)))}
+    return {"workload": "[2,2,12], 16 qubit branches, 20 seeded cd/cr; per-branch Wigner and Husimi on "
+                        "512x512 over [-6,6]^2, native traced readouts, joint m(0,1,2)",
+            "mixed": res["mixed"], "complex128": res["complex128"], "mixed_vs_fp64": err}
+
+
+def c1_section(steps):
+    from hipcvdv import HipCvdv, SeparableState
+    from oracle.cvdv_oracle import CvdvOracle
+    from oracle.separable_np import SeparableStateNP
+
+    grid = np.linspace(-5, 5, 201)
+
+    def circuit(sim):
+        sim.h(0, 0)
+        sim.cd(1, 0, 0, 1.5)
+        sim.r(1, 0.7)
+        sim.s(1, 0.3)
+        sim.h(0, 0)
+
+    h = HipCvdv([1, 10])
+    sep = SeparableState([1, 10])
+    sep.setZero(0)
+    sep.setCoherent(1, 2 + 1j)
+
+    def gpu():
+        h.initStateVector(sep)
+        circuit(h)
+        return h.m(0), h.getWignerGrid(1, grid, grid)
+
+    gpu()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        m0, W = gpu()
+    t_gpu = (time.perf_counter() - t0) / steps
+    o = CvdvOracle([1, 10], convention="cuda")
+    sepn = SeparableStateNP([1, 10])
+    sepn.setZero(0)
+    sepn.setCoherent(1, 2 + 1j)
+    t0 = time.perf_counter()
+    o.initStateVector(sepn)
+    circuit(o)
+    mo, Wo = o.m(0), o.getWignerGrid(1, grid, grid)
+    t_cpu = time.perf_counter() - t0
+    return {"workload": "[1,10] complex128: h, cd(1.5), r(0.7), s(0.3), h; m(0) and the Wigner of the "
+                        "oscillator on 201x201 over [-5,5]^2",
+            "gpu_ms": t_gpu * 1e3, "cpu_oracle_ms": t_cpu * 1e3, "cpu_cores": 1,
+            "max_abs_err_wigner": float(np.max(np.abs(W - Wo))), "max_abs_wigner": float(np.max(np.abs(Wo))),
+            "max_abs_err_m0": float(np.max(np.abs(np.asarray(m0) - np.asarray(mo))))}
+
+
 def run_gpu(prec, ops, steps, warmup):
     import torch
     from hipcvdv import HipCvdv, SeparableState
@@ -432,6 +539,7 @@ def main():
     ap.add_argument("--c4-seq", type=int, default=64, help="circuits timed one by one")
     ap.add_argument("--c4-only", action="store_true", help="run only the batched-circuit workload")
     ap.add_argument("--no-c3", action="store_true", help="skip the state-transfer workload")
+    ap.add_argument("--no-c5", action="store_true", help="skip the phase-space readout workloads (C1, C5)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the state vector is not sharded across GPUs.",
@@ -501,6 +609,9 @@ def main():
         out["norm_mixed"] = n1
     if not args.no_c3 and not args.only:
         out["c3"] = c3_section(max(1, args.steps))
+    if not args.no_c5 and not args.only:
+        out["c1"] = c1_section(max(1, args.steps))
+        out["c5"] = c5_section(max(1, args.steps))
     if not args.no_c4 and not args.only:
         out["c4"] = c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))
     if not args.no_cpu_baseline:

@@ -631,6 +631,8 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     using C = typename CT<R>::type;
     const uint64_t N = 1ull << nb, S = 1ull << sh;
     const uint64_t H = total / (N * S);
+    if (H == 0 || (H & (H - 1)) != 0 || H * N * S != total)
+        throw CvdvError{CVDV_ERR_ARG, "FFT over a buffer whose size is not a power-of-two multiple of the field"};
     bool act = false, folded = false;
     const FastDiag pre = compile_diag(ctx, preProg, act, outScale, &folded);
     const int usePre = act ? 1 : 0;

@@ -238,17 +238,21 @@ static void get_wigner_grid(CVDVContext* ctx, int r, const double* xs, int nx, c
     const double dx = ctx->dx[r];
     const uint64_t A = ctx->totalSize, S = A >> n;
     const uint64_t outBlocks = perSlice ? S : 1;
-    int nxc = nx;
-    while (nxc > 1 && ((uint64_t)nxc * S * N > GRID_CHUNK || (uint64_t)nxc * nd * outBlocks > GRID_CHUNK))
-        nxc = (nxc + 1) / 2;
-    std::vector<int> hi(nx);
-    std::vector<double> ht(nx);
+    // x chunk: a power of two (the batched FFT over the chunk's S rows per point
+    // needs a power-of-two row count); the last chunk is padded with t = 0 rows
+    int nxc = 1;
+    while (nxc < nx && (uint64_t)nxc * 2 * S * N <= GRID_CHUNK && (uint64_t)nxc * 2 * nd * outBlocks <= GRID_CHUNK)
+        nxc *= 2;
+    const int nChunks = (nx + nxc - 1) / nxc;
+    const int nxPad = nChunks * nxc;
+    std::vector<int> hi(nxPad, 0);
+    std::vector<double> ht(nxPad, 0.0);
     for (int x = 0; x < nx; ++x) grid_split(xs[x], N, dx, hi[x], ht[x]);
     // scratch: X | spec | Y (nxc S N) | tmp (nxc S N) | Apairs (outBlocks nxc nd) | E (np nd) | Z (outBlocks np nx)
     //          | ix | tx | ps | out
     const uint64_t nY = (uint64_t)nxc * S * N, nA = outBlocks * nxc * nd, nE = (uint64_t)np * nd;
     const uint64_t nZ = outBlocks * (uint64_t)np * nx;
-    const size_t bytes = (2 * A + 2 * nY + nA + nE + nZ) * 16 + (size_t)nx * 12 + (size_t)np * 8 + nZ * 8 + 256;
+    const size_t bytes = (2 * A + 2 * nY + nA + nE + nZ) * 16 + (size_t)nxPad * 12 + (size_t)np * 8 + nZ * 8 + 256;
     char* base = (char*)ctx_scratch(ctx, bytes);
     double2* X = (double2*)base;
     double2* spec = X + A;
@@ -259,7 +263,7 @@ static void get_wigner_grid(CVDVContext* ctx, int r, const double* xs, int nx, c
     double2* Z = E + nE;
     double* dps = (double*)(Z + nZ);
     double* dtx = dps + np;
-    double* dout = dtx + nx;
+    double* dout = dtx + nxPad;
     int* dix = (int*)(dout + nZ);
     grid_slices(ctx, r, X);
     upload(ctx, dps, std::vector<double>(ps, ps + np));
@@ -272,8 +276,8 @@ static void get_wigner_grid(CVDVContext* ctx, int r, const double* xs, int nx, c
     HIPCHK(hipGetLastError());
     for (int x0 = 0; x0 < nx; x0 += nxc) {
         const int c = nx - x0 < nxc ? nx - x0 : nxc;
-        const uint64_t ny = (uint64_t)c * S * N;
-        hipLaunchKernelGGL(k_grid_shift, dim3(grid_for(ny)), dim3(256), 0, ctx->stream, Y, spec, S, n, dtx + x0, c);
+        const uint64_t ny = (uint64_t)nxc * S * N;  // padded rows keep the FFT's row count a power of two
+        hipLaunchKernelGGL(k_grid_shift, dim3(grid_for(ny)), dim3(256), 0, ctx->stream, Y, spec, S, n, dtx + x0, nxc);
         HIPCHK(hipGetLastError());
         if (n > 0) fft_generic_c128(ctx, Y, tmp, ny, 0, n, true, 1.0);
         const uint64_t na = (perSlice ? S : 1) * (uint64_t)c * nd;

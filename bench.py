@@ -525,6 +525,43 @@ def cpu_baseline(ops, budget_s):
                       f"({passes} passes) of the 200-gate circuit in {el:.1f} s, scipy.fft workers={workers}"}
 
 
+def copy_probe(reps=10):
+    """Plain device copy of one complex64 state (2^28 amplitudes, 2 GiB read + 2 GiB
+    written per copy): the achievable-bandwidth reference for the FFT passes, and
+    the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+
+    A = 1 << sum(REGS)
+    src = torch.zeros(A, dtype=torch.complex64, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del src, dst
+    torch.cuda.empty_cache()
+    return {"bytes_per_copy": 2 * A * 8, "ms": ms, "gbps": 2 * A * 8 / (ms * 1e-3) / 1e9}
+
+
+def pmc_traffic():
+    """HBM bytes per FFT launch from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE, each scaled by the same counter on a plain copy of known size)."""
+    path = os.path.join(ROOT, "profiles", "k_fft_tile_c2_mixed_pmc.txt")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -540,12 +577,17 @@ def main():
     ap.add_argument("--c4-only", action="store_true", help="run only the batched-circuit workload")
     ap.add_argument("--no-c3", action="store_true", help="skip the state-transfer workload")
     ap.add_argument("--no-c5", action="store_true", help="skip the phase-space readout workloads (C1, C5)")
+    ap.add_argument("--copy-only", action="store_true",
+                    help="only the plain-copy bandwidth probe (counter calibration runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the state vector is not sharded across GPUs.",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_only:
+        print(json.dumps({"copy": copy_probe()}))
+        return
     if args.c4_only:
         print(json.dumps(c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))))
         return
@@ -572,6 +614,7 @@ def main():
     fft_launches = r["prof"]["fft_launches"]
     fft_ms = r["prof"]["fft_ms"]
     achieved = fft_alg_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else None
+    copy = copy_probe()
     out = {
         "metric": METRIC,
         "value": value,
@@ -592,7 +635,9 @@ def main():
                    else "complex128"},
         "pct_hbm_peak": value * pass_bytes / 1e9 / HBM_PEAK_GBS * 100.0,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": pmc_traffic(),
+                     "copy_gbps": copy["gbps"],
+                     "frac_of_copy": (achieved / copy["gbps"]) if achieved else None,
                      "kernel": "k_fft_tile", "fft_launches": fft_launches,
                      "avg_launch_ms": fft_ms / fft_launches if fft_launches else None,
                      "algorithmic_bytes_per_launch": fft_alg_bytes / fft_launches if fft_launches else None},

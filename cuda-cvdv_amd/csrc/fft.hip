@@ -446,6 +446,55 @@ struct PreLds {
     DiagTerm gen[CVDV_FAST_GEN];
 };
 
+// The generic terms over a one-row tile (only the transformed field
+// [fs, fs + fn) varies inside it): when every term is affine in that field's
+// coordinate m, their sum is a·m + b (mod 1) with per-tile a, b, and one FP64
+// fma per element replaces the per-term evaluation.  a is reduced mod 1 (m is
+// an integer), so a·m stays below 2^(fn-1) and keeps ~2^-40 absolute accuracy.
+struct GenFold {
+    double a, b;
+    int ok;
+};
+
+__device__ __forceinline__ bool field_overlaps(int sh, int nb, int fs, int fn) {
+    return sh < fs + fn && fs < sh + nb;
+}
+
+__device__ __forceinline__ GenFold fold_generic(const PreLds& P, const FastDiag& pre, uint64_t g0, int fs, int fn) {
+    GenFold f{0.0, pre.genrev, 1};
+    for (int i = 0; i < pre.ngen; ++i) {
+        const DiagTerm& T = P.gen[i];
+        if (T.ctrl >= 0 && T.ctrl >= fs && T.ctrl < fs + fn) f.ok = 0;
+        const double sg = (T.ctrl >= 0 && ((g0 >> T.ctrl) & 1ull)) ? -1.0 : 1.0;
+        const bool in1 = T.sh1 == fs && T.nb1 == fn;
+        if (T.kind == DT_BILIN) {
+            const bool in2 = T.sh2 == fs && T.nb2 == fn;
+            if ((!in1 && field_overlaps(T.sh1, T.nb1, fs, fn)) || (!in2 && field_overlaps(T.sh2, T.nb2, fs, fn)) ||
+                (in1 && in2))
+                f.ok = 0;
+            else if (in1)
+                f.a += cvdv_frac(sg * T.c1 * diag_field_m(g0, T.sh2, T.nb2));
+            else if (in2)
+                f.a += cvdv_frac(sg * T.c1 * diag_field_m(g0, T.sh1, T.nb1));
+            else
+                f.b += cvdv_frac(sg * diag_term_rev(T, g0));
+        } else if (T.kind == DT_POLY) {
+            if (in1 && T.c2 == 0.0 && T.c3 == 0.0)
+                f.a += cvdv_frac(sg * T.c1);
+            else if (field_overlaps(T.sh1, T.nb1, fs, fn))
+                f.ok = 0;
+            else
+                f.b += cvdv_frac(sg * diag_term_rev(T, g0));
+        } else {
+            if (T.sh1 >= fs && T.sh1 < fs + fn) f.ok = 0;
+            else f.b += cvdv_frac(sg * diag_term_rev(T, g0));
+        }
+    }
+    f.a = cvdv_frac(f.a);
+    f.b = cvdv_frac(f.b);
+    return f;
+}
+
 // Phase of the generic (untabulated) terms at element g.
 template <typename C>
 __device__ __forceinline__ C gen_factor(const PreLds& P, const FastDiag& pre, uint64_t g) {
@@ -504,10 +553,28 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         }
         if (pre.ngen > 0) {
             asm volatile("" : "+v"(tt));
+            // one-row tiles: the terms are affine in the row coordinate
+            GenFold gf{0.0, 0.0, 0};
+            if (LGC == 0 && d.lgHb == 0) gf = fold_generic(P, pre, inBase, d.lgInR, LGM);
+            if (gf.ok) {
+                using R = decltype(C{}.x);
 #pragma unroll
-            for (int k = 0; k < NB; ++k)
+                for (int k = 0; k < NB; ++k) {
+                    const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
 #pragma unroll
-                for (int i = 0; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], gen_factor<C>(P, pre, elem(k, i)));
+                    for (int i = 0; i < RP; ++i) {
+                        const int q = p.q + (i << lgMq);
+                        const double m = (double)(2 * q + 1 - (1 << LGM));
+                        x[k * RP + i] = cmul(x[k * RP + i], unit_rev_fast(cvdv_frac(fma(gf.a, m, gf.b)), R(0)));
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < NB; ++k)
+#pragma unroll
+                    for (int i = 0; i < RP; ++i)
+                        x[k * RP + i] = cmul(x[k * RP + i], gen_factor<C>(P, pre, elem(k, i)));
+            }
         }
         if (d.conjIn) {
 #pragma unroll
@@ -600,8 +667,16 @@ __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ t
 }
 
 // PF: persistent workgroups that prefetch the next tile (one-row tiles).
+// Workgroup size bound: one-row tiles of 2^12 points and more take up to 1024
+// threads; every other tile fits 64 KiB of LDS, i.e. at most 512 (complex64) or
+// 256 (complex128) threads at 8 elements each.
+template <typename R, int LGM, int LGC>
+constexpr int tile_threads_max() {
+    return (LGC == 0 && LGM >= 12) ? 1024 : (sizeof(R) == 4 ? 512 : 256);
+}
+
 template <typename R, int LGM, int LGC, int E, bool PF>
-__global__ void __launch_bounds__(1024)
+__global__ void __launch_bounds__((tile_threads_max<R, LGM, LGC>()))
 k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d, FastDiag pre,
            const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
     using C = typename CT<R>::type;
@@ -879,6 +954,8 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
                         const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
     if constexpr (tile_instance<R, LGM, LGC>()) {
         auto go = [&](auto kfn, int E, bool persistent) {
+            if (tile / E > tile_threads_max<R, LGM, LGC>())
+                throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile larger than its kernel's workgroup bound"};
             if (lds > 64 * 1024)
                 HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             // persistent instances: as many workgroups as the device holds at once

@@ -106,7 +106,10 @@ def _random_ops(rng, regs, n):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("regs,seed", [([2, 6, 6], 11), ([1, 9], 12), ([3, 5, 5], 13), ([2, 8, 4], 14),
-                                       ([2, 11], 15), ([1, 7, 7], 16)])
+                                       ([2, 11], 15), ([1, 7, 7], 16),
+                                       # one-row FFT tiles of the last register (2^11; 2^13 persistent)
+                                       # with bilinear terms to the other oscillator pending
+                                       ([1, 4, 11], 17), ([1, 2, 13], 18)])
 def test_random_circuit_parity(regs, seed, prec):
     print(f"[gpu] random {regs} seed {seed} {prec}", flush=True)
     rng = np.random.default_rng(seed)

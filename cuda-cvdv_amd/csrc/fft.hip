@@ -534,19 +534,33 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             return inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c +
                    ((uint64_t)(i << lgMq) << d.lgInR);
         };
-        // table-major: the E entries of one table are loaded together, so a wave
-        // has E loads in flight instead of one dependent load per element
-        for (int t = 0; t < pre.ntab; ++t) {
-            // opaque: the element indices are recomputed per table, not held
-            asm volatile("" : "+v"(tt));
-            const DiagTab T = P.tab[t];
+        // Table-major: the E entries of one table are loaded together, so a wave
+        // has E loads in flight instead of one dependent load per element.  The
+        // first table is applied in straight-line code, so its loads issue with
+        // the tile's input loads and share their wait (a loop that touches x
+        // would wait for x's loads before issuing its first table load).
+        if (pre.ntab > 0) {
+            const DiagTab T = P.tab[0];
             constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
 #pragma unroll
             for (int j0 = 0; j0 < E; j0 += CH) {
                 C e[CH];
 #pragma unroll
-                for (int j = 0; j < CH; ++j)
-                    e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+                for (int j = 0; j < CH; ++j) e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+#pragma unroll
+                for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
+            }
+        }
+        for (int t = 1; t < pre.ntab; ++t) {
+            // opaque: the element indices are recomputed per table, not held
+            asm volatile("" : "+v"(tt));
+            const DiagTab T = P.tab[t];
+            constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;
+#pragma unroll
+            for (int j0 = 0; j0 < E; j0 += CH) {
+                C e[CH];
+#pragma unroll
+                for (int j = 0; j < CH; ++j) e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
 #pragma unroll
                 for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
             }
@@ -593,15 +607,6 @@ __device__ __forceinline__ C tw4_factor(const FftPass& d, uint64_t col, int row,
     return tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)];
 }
 
-// Holds the caller until v has landed.  The memory counter completes loads and
-// stores in issue order, so a load waited for after a store also waits for that
-// store: the four-step twiddles of a tile are all loaded, and waited for, before
-// its first store.
-template <typename C>
-__device__ __forceinline__ void landed(const C& v) {
-    asm volatile("" ::"v"(v.x), "v"(v.y) : "memory");
-}
-
 // Output conjugation (inverse) and scale.
 template <typename R, typename C>
 __device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleLo) {
@@ -613,12 +618,27 @@ __device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleL
     return v;
 }
 
+// Four-step twiddles of the outputs a thread stores directly from the last stage
+// (rows q + m*M/RP of column c), in that order.
+template <int RP, int E, int LGM, int LGC, typename C>
+__device__ __forceinline__ void tw4_direct(C* f, const FftPass& d, uint64_t lo0, const C* __restrict__ tw4, int tid,
+                                           int nt) {
+    constexpr int NB = E / RP;
+    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
+    constexpr int lgMq = LGM - LGR;
+#pragma unroll
+    for (int k = 0; k < NB; ++k) {
+        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+#pragma unroll
+        for (int m = 0; m < RP; ++m) f[k * RP + m] = tw4_factor(d, lo0 + p.c, p.q + (m << lgMq), tw4);
+    }
+}
+
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
-// TW4: the pass may carry four-step twiddles.
+// TW4: the pass may carry four-step twiddles, f = tw4_direct's factors.
 template <int RP, int E, int LGM, int LGC, bool TW4, typename R, typename C>
 __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
-                                           uint64_t lo0, const C* __restrict__ tw4, R scale, R scaleLo, int tid,
-                                           int nt) {
+                                           const C* f, R scale, R scaleLo, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
@@ -626,15 +646,6 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, cons
 #pragma unroll
     for (int j = 0; j < E; ++j) v[j] = x[j];
     if (TW4 && d.tw4) {
-        C f[E];
-#pragma unroll
-        for (int k = 0; k < NB; ++k) {
-            const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
-#pragma unroll
-            for (int m = 0; m < RP; ++m) f[k * RP + m] = tw4_factor(d, lo0 + p.c, p.q + (m << lgMq), tw4);
-        }
-#pragma unroll
-        for (int j = 0; j < E; ++j) landed(f[j]);
 #pragma unroll
         for (int j = 0; j < E; ++j) v[j] = cmul(v[j], f[j]);
     }
@@ -742,25 +753,33 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         lds_barrier();
     };
     // Staged store, second half: LDS to global memory in the output's contiguous order.
-    auto store_lds = [&](uint64_t t, int ti, const C* tw4t) {
+    // Four-step twiddles of the outputs thread ti stores for tile t, in store
+    // order, all loaded before the first store (a load waited for after a store
+    // would also wait for that store: the memory counter completes in order).
+    auto tw4_load = [&](C* f, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        // a thread stores elements ti + j*nt (pairs 2ti + 2j*nt), j < E (E/2): nt = tile / E
-        C f[E];
-        if (d.tw4) {
-#pragma unroll
-            for (int j = 0; j < E; ++j) {
-                const int e = (sizeof(C) == 8 && d.vecOut) ? 2 * ti + (j >> 1) * 2 * nt + (j & 1) : ti + j * nt;
-                int c, row, hb;
-                tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-                f[j] = tw4_factor(d, lo0 + c, row, tw4t);
-            }
-#pragma unroll
-            for (int j = 0; j < E; ++j) landed(f[j]);
+        if (d.directOut) {
+            tw4_direct<RL, E, LGM, LGC>(f, d, lo0, tw4, ti, nt);
+            return;
         }
+        // a thread stores elements ti + j*nt (pairs 2ti + 2j*nt), j < E (E/2): nt = tile / E
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const int e = (sizeof(C) == 8 && d.vecOut) ? 2 * ti + (j >> 1) * 2 * nt + (j & 1) : ti + j * nt;
+            int c, row, hb;
+            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
+            f[j] = tw4_factor(d, lo0 + c, row, tw4);
+        }
+    };
+    auto store_lds = [&](uint64_t t, int ti, const C* f) {
+        uint64_t lo0, inBase, outBase;
+        tile_base(t, lo0, inBase, outBase);
         auto value = [&](int c, int row, int hb, int j) {
             C v = lds[lds_ix((((hb << LGM) + row) << lgC) + c)];
-            if (d.tw4) v = cmul(v, f[j]);
+            if constexpr (LGC > 0) {
+                if (d.tw4) v = cmul(v, f[j]);
+            }
             return out_finish(v, d, scale, scaleLo);
         };
         if (sizeof(C) == 8 && d.vecOut) {
@@ -785,15 +804,15 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             }
         }
     };
-    auto store = [&](C* x, uint64_t t, int ti, const C* tw4t) {
+    auto store = [&](C* x, uint64_t t, int ti, const C* f) {
         if (d.directOut) {
             uint64_t lo0, inBase, outBase;
             tile_base(t, lo0, inBase, outBase);
-            last_store<RL, E, LGM, LGC, !PF>(out, x, d, outBase, lo0, tw4t, scale, scaleLo, ti, nt);
+            last_store<RL, E, LGM, LGC, (LGC > 0)>(out, x, d, outBase, f, scale, scaleLo, ti, nt);
             return;
         }
         stage_out(x, ti);
-        store_lds(t, ti, tw4t);
+        store_lds(t, ti, f);
     };
 
     if constexpr (!PF) {
@@ -805,7 +824,11 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
         prephase(x, t, tid);
         stages(x, tid, twl);
-        store(x, t, tid, tw4);
+        C f[E];
+        if constexpr (LGC > 0) {
+            if (d.tw4) tw4_load(f, t, tid);
+        }
+        store(x, t, tid, f);
     } else {
         // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
         // tile's inputs are loaded into registers before the current tile is
@@ -842,7 +865,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
                 stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, ti, nt);
             }
             stages(x, ti, twt);
-            store(x, t, ti, tw4);
+            store(x, t, ti, nullptr);  // one-row tiles carry no four-step twiddles
             lds_barrier();  // the next tile's first exchange overwrites LDS
             t = tn;
         }
@@ -967,8 +990,9 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
         // and keep their (split) twiddles in LDS
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
+        // four-step first passes always span several columns (N1 * S >= 2)
+        if (LGC == 0 && d.tw4) throw CvdvError{CVDV_ERR_UNSUPPORTED, "four-step twiddles on a one-column FFT tile"};
         if constexpr (PF) {
-            if (d.tw4) throw CvdvError{CVDV_ERR_UNSUPPORTED, "four-step twiddles on a one-row FFT tile"};
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
         }

@@ -277,7 +277,7 @@ struct FftPass {
     int rowFastOut;                // linear order of the staged store after the column index
     int vecOut;                    // 1: pairs of complex64 stored as one 16-byte access
     int directOut;                 // 1: last stage stores straight to global (coalesced rows)
-    int tw4, tw4Shift, tw4LogN;
+    int tw4, tw4Shift, tw4HiBits, tw4LogN;
     int usePre, useScale;
     int conjIn, conjOut;           // inverse = conj . forward . conj, split over the passes
     int lgInH1, lgInH2, lgInR;     // every stride is a power of two
@@ -520,13 +520,57 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
     }
 }
 
-// Pending phases (and the input conjugation of an inverse) on the loaded inputs.
-template <int RP, int E, int LGM, int LGC, typename C>
+// Pending phases, the four-step twiddle W_N^{j1 k2} of a second pass (row j1 =
+// the transformed index, k2 = (g >> tw4Shift) mod 2^tw4HiBits; TW4: the
+// instance may carry one) and the input conjugation of an inverse on the loaded
+// inputs.
+template <int RP, int E, int LGM, int LGC, bool TW4, typename C>
 __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
-                                           const FastDiag& pre, int tid, int nt) {
+                                           const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
+    if (TW4 && d.tw4) {
+        // second four-step pass: straight-line, so these loads issue with the
+        // tile's input loads and share their wait
+        if (d.lgHb == 0) {
+            // one row block per tile: k2 is the tile's, so W^{k2 (q + i M/RP)} =
+            // W^{k2 q} W^{k2 i M/RP} takes one gathered and RP-1 tile-uniform
+            // (scalar) loads per butterfly instead of RP gathers
+            const uint32_t nmask = (1u << d.tw4LogN) - 1u;
+            const uint32_t k2 = (uint32_t)(inBase >> d.tw4Shift) & ((1u << d.tw4HiBits) - 1u);
+            C st[RP];
+#pragma unroll
+            for (int i = 1; i < RP; ++i) st[i] = tw4[(k2 * (uint32_t)(i << lgMq)) & nmask];
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+                const C b = tw4[(k2 * (uint32_t)p.q) & nmask];
+                x[k * RP] = cmul(x[k * RP], b);
+#pragma unroll
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(b, st[i]));
+            }
+        } else {
+            // k2 differs between row blocks: per butterfly, gather W^{k2 q} and
+            // w = W^{k2 M/RP}; the powers w^i come from a multiplication tree of
+            // depth log2(RP)
+#pragma unroll
+            for (int k = 0; k < NB; ++k) {
+                const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
+                const uint64_t g0 = inBase + ((uint64_t)p.hb << d.lgInH2) + ((uint64_t)p.q << d.lgInR) + p.c;
+                const uint32_t nmask = (1u << d.tw4LogN) - 1u;
+                const uint32_t k2 = (uint32_t)(g0 >> d.tw4Shift) & ((1u << d.tw4HiBits) - 1u);
+                const C b = tw4[(k2 * (uint32_t)p.q) & nmask];
+                C st[RP];
+                st[1] = tw4[(k2 << lgMq) & nmask];
+#pragma unroll
+                for (int i = 2; i < RP; ++i) st[i] = cmul(st[i / 2], st[i - i / 2]);
+                x[k * RP] = cmul(x[k * RP], b);
+#pragma unroll
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(b, st[i]));
+            }
+        }
+    }
     if (d.usePre) {
         int tt = tid;
         auto elem = [&](int k, int i) {
@@ -600,13 +644,6 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
     }
 }
 
-// Four-step twiddle W_N^{j1 row} of output (col, row), j1 = col >> tw4Shift.
-template <typename C>
-__device__ __forceinline__ C tw4_factor(const FftPass& d, uint64_t col, int row, const C* __restrict__ tw4) {
-    const uint32_t j1 = (uint32_t)(col >> d.tw4Shift);
-    return tw4[(j1 * (uint32_t)row) & ((1u << d.tw4LogN) - 1u)];
-}
-
 // Output conjugation (inverse) and scale.
 template <typename R, typename C>
 __device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleLo) {
@@ -618,37 +655,13 @@ __device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleL
     return v;
 }
 
-// Four-step twiddles of the outputs a thread stores directly from the last stage
-// (rows q + m*M/RP of column c), in that order.
-template <int RP, int E, int LGM, int LGC, typename C>
-__device__ __forceinline__ void tw4_direct(C* f, const FftPass& d, uint64_t lo0, const C* __restrict__ tw4, int tid,
-                                           int nt) {
-    constexpr int NB = E / RP;
-    constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
-    constexpr int lgMq = LGM - LGR;
-#pragma unroll
-    for (int k = 0; k < NB; ++k) {
-        const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
-#pragma unroll
-        for (int m = 0; m < RP; ++m) f[k * RP + m] = tw4_factor(d, lo0 + p.c, p.q + (m << lgMq), tw4);
-    }
-}
-
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
-// TW4: the pass may carry four-step twiddles, f = tw4_direct's factors.
-template <int RP, int E, int LGM, int LGC, bool TW4, typename R, typename C>
-__device__ __forceinline__ void last_store(C* __restrict__ out, const C* x, const FftPass& d, uint64_t outBase,
-                                           const C* f, R scale, R scaleLo, int tid, int nt) {
+template <int RP, int E, int LGM, int LGC, typename R, typename C>
+__device__ __forceinline__ void last_store(C* __restrict__ out, const C* v, const FftPass& d, uint64_t outBase,
+                                           R scale, R scaleLo, int tid, int nt) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
-    C v[E];
-#pragma unroll
-    for (int j = 0; j < E; ++j) v[j] = x[j];
-    if (TW4 && d.tw4) {
-#pragma unroll
-        for (int j = 0; j < E; ++j) v[j] = cmul(v[j], f[j]);
-    }
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
@@ -715,7 +728,6 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
 
     const R scale = (R)d.outScale;
     const R scaleLo = (R)(d.outScale - (double)scale);
-    (void)tw4;
     // One-row tiles keep every twiddle in LDS after the tile (split table, ~11 KiB):
     // no global loads remain between a prefetch and its use.
     const C* twl = tw;
@@ -739,7 +751,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     auto prephase = [&](C* x, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        stage0_pre<R0, E, LGM, LGC>(x, d, inBase, P, pre, ti, nt);
+        stage0_pre<R0, E, LGM, LGC, !PF>(x, d, inBase, P, pre, tw4, ti, nt);
     };
     // Transform the phased tile in registers (the last stage's outputs stay in x).
     auto stages = [&](C* x, int ti, const C* twt) {
@@ -753,34 +765,11 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         lds_barrier();
     };
     // Staged store, second half: LDS to global memory in the output's contiguous order.
-    // Four-step twiddles of the outputs thread ti stores for tile t, in store
-    // order, all loaded before the first store (a load waited for after a store
-    // would also wait for that store: the memory counter completes in order).
-    auto tw4_load = [&](C* f, uint64_t t, int ti) {
+    auto store_lds = [&](uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        if (d.directOut) {
-            tw4_direct<RL, E, LGM, LGC>(f, d, lo0, tw4, ti, nt);
-            return;
-        }
-        // a thread stores elements ti + j*nt (pairs 2ti + 2j*nt), j < E (E/2): nt = tile / E
-#pragma unroll
-        for (int j = 0; j < E; ++j) {
-            const int e = (sizeof(C) == 8 && d.vecOut) ? 2 * ti + (j >> 1) * 2 * nt + (j & 1) : ti + j * nt;
-            int c, row, hb;
-            tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-            f[j] = tw4_factor(d, lo0 + c, row, tw4);
-        }
-    };
-    auto store_lds = [&](uint64_t t, int ti, const C* f) {
-        uint64_t lo0, inBase, outBase;
-        tile_base(t, lo0, inBase, outBase);
-        auto value = [&](int c, int row, int hb, int j) {
-            C v = lds[lds_ix((((hb << LGM) + row) << lgC) + c)];
-            if constexpr (LGC > 0) {
-                if (d.tw4) v = cmul(v, f[j]);
-            }
-            return out_finish(v, d, scale, scaleLo);
+        auto value = [&](int c, int row, int hb) {
+            return out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, scale, scaleLo);
         };
         if (sizeof(C) == 8 && d.vecOut) {
 #pragma unroll
@@ -789,8 +778,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
                 int c, row, hb, c1, row1, hb1;
                 tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
                 tile_coords<LGM>(e + 1, lgC, lgHb, d.rowFastOut, c1, row1, hb1);
-                const C v0 = value(c, row, hb, 2 * j);
-                const C v1 = value(c1, row1, hb1, 2 * j + 1);
+                const C v0 = value(c, row, hb);
+                const C v1 = value(c1, row1, hb1);
                 const uint64_t g = outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c;
                 *reinterpret_cast<float4*>(out + g) = make_float4((float)v0.x, (float)v0.y, (float)v1.x, (float)v1.y);
             }
@@ -800,19 +789,19 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
                 const int e = ti + j * nt;
                 int c, row, hb;
                 tile_coords<LGM>(e, lgC, lgHb, d.rowFastOut, c, row, hb);
-                out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = value(c, row, hb, j);
+                out[outBase + ((uint64_t)hb << d.lgOutH2) + ((uint64_t)row << d.lgOutR) + c] = value(c, row, hb);
             }
         }
     };
-    auto store = [&](C* x, uint64_t t, int ti, const C* f) {
+    auto store = [&](C* x, uint64_t t, int ti) {
         if (d.directOut) {
             uint64_t lo0, inBase, outBase;
             tile_base(t, lo0, inBase, outBase);
-            last_store<RL, E, LGM, LGC, (LGC > 0)>(out, x, d, outBase, f, scale, scaleLo, ti, nt);
+            last_store<RL, E, LGM, LGC>(out, x, d, outBase, scale, scaleLo, ti, nt);
             return;
         }
         stage_out(x, ti);
-        store_lds(t, ti, f);
+        store_lds(t, ti);
     };
 
     if constexpr (!PF) {
@@ -824,11 +813,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
         prephase(x, t, tid);
         stages(x, tid, twl);
-        C f[E];
-        if constexpr (LGC > 0) {
-            if (d.tw4) tw4_load(f, t, tid);
-        }
-        store(x, t, tid, f);
+        store(x, t, tid);
     } else {
         // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
         // tile's inputs are loaded into registers before the current tile is
@@ -865,7 +850,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
                 stage0_load<R0, E, LGM, LGC>(in, xn, d, inBase, ti, nt);
             }
             stages(x, ti, twt);
-            store(x, t, ti, nullptr);  // one-row tiles carry no four-step twiddles
+            store(x, t, ti);
             lds_barrier();  // the next tile's first exchange overwrites LDS
             t = tn;
         }
@@ -913,9 +898,13 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     // 256 contiguous bytes per row: the measured HBM rate of strided tiles rises
     // from ~4.0 TB/s with 128-byte rows to ~5.1 TB/s with 256-byte rows
     const uint64_t W = prec == CVDV_C128 ? 16 : 32;
-    const uint64_t C = g.nLo < W ? g.nLo : W;
-    const uint64_t tileCM = C * (uint64_t)g.M;
+    uint64_t C = g.nLo < W ? g.nLo : W;
     const size_t small = 64 * 1024;
+    // the passes of a two-step split narrow their rows before giving up (a tall
+    // pass of a contiguous register above one tile has M up to 2^12 rows)
+    if (!allowBig)
+        while (C > 1 && lds_bytes(C * (uint64_t)g.M, esz) > small) C /= 2;
+    const uint64_t tileCM = C * (uint64_t)g.M;
     const bool big = allowBig && C == 1 && lds_bytes(tileCM, esz) <= device_max_lds() && tileCM <= 16384;
     if (lds_bytes(tileCM, esz) > small && !big) return false;
     uint64_t Hb = 1;
@@ -989,20 +978,22 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         };
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
         // and keep their (split) twiddles in LDS
+        // (the second pass of a four-step split, which carries the four-step
+        // twiddle, runs one tile per workgroup)
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
-        // four-step first passes always span several columns (N1 * S >= 2)
-        if (LGC == 0 && d.tw4) throw CvdvError{CVDV_ERR_UNSUPPORTED, "four-step twiddles on a one-column FFT tile"};
-        if constexpr (PF) {
+        if (PF && !d.tw4) {
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
         }
         if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
             if (tile / 8 > 1024) {
-                go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
+                if (!d.tw4) go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
+                else go(k_fft_tile<R, LGM, LGC, 16, false>, 16, false);
                 return;
             }
         }
-        go(k_fft_tile<R, LGM, LGC, 8, PF>, 8, PF);
+        if (PF && !d.tw4) go(k_fft_tile<R, LGM, LGC, 8, PF>, 8, PF);
+        else go(k_fft_tile<R, LGM, LGC, 8, false>, 8, false);
     } else {
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile shape without a kernel instance"};
     }
@@ -1069,7 +1060,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
     FftPass d{};
     if (fit_pass(g1, prec, d, true)) {
-        d.tw4 = 0; d.tw4Shift = 0; d.tw4LogN = 0;
+        d.tw4 = 0; d.tw4Shift = 0; d.tw4HiBits = 0; d.tw4LogN = 0;
         d.usePre = usePre;
         d.useScale = folded ? 0 : 1;
         d.conjIn = inv;
@@ -1106,14 +1097,16 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     FftPass d2{};
     if (!fit_pass(p1, prec, d1, false) || !fit_pass(p2, prec, d2, false))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
-    // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its output
-    d1.tw4 = 1; d1.tw4Shift = sh; d1.tw4LogN = nb;
+    // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its
+    // output; the twiddle T = W_N^{j1 k2} is applied by pass 2 on load, where the
+    // pass has memory time to spare (pass 1 also applies the pending phases)
+    d1.tw4 = 0; d1.tw4Shift = 0; d1.tw4HiBits = 0; d1.tw4LogN = 0;
     d1.usePre = usePre;
     d1.useScale = 0;
     d1.conjIn = inv;
     d1.conjOut = 0;
     d1.outScale = 1.0;
-    d2.tw4 = 0; d2.tw4Shift = 0; d2.tw4LogN = 0;
+    d2.tw4 = 1; d2.tw4Shift = sh + n1; d2.tw4HiBits = n2; d2.tw4LogN = nb;
     d2.usePre = 0;
     d2.useScale = folded ? 0 : 1;
     d2.conjIn = 0;

@@ -109,7 +109,10 @@ def _random_ops(rng, regs, n):
                                        ([2, 11], 15), ([1, 7, 7], 16),
                                        # one-row FFT tiles of the last register (2^11; 2^13 persistent)
                                        # with bilinear terms to the other oscillator pending
-                                       ([1, 4, 11], 17), ([1, 2, 13], 18)])
+                                       ([1, 4, 11], 17), ([1, 2, 13], 18),
+                                       # a contiguous register above one tile: two-pass FFT whose
+                                       # second pass has one column and carries the twiddle
+                                       ([1, 15], 19)])
 def test_random_circuit_parity(regs, seed, prec):
     print(f"[gpu] random {regs} seed {seed} {prec}", flush=True)
     rng = np.random.default_rng(seed)

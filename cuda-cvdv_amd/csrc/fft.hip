@@ -193,7 +193,8 @@ const void* fft_stage_table(TwiddleCache*& tcp, int logM, int prec) {
 // (radix R, sub-length L): L <= 64 stores W_{RL}^{p i} directly (p < L); larger L
 // stores W_{RL/64}^{a i} (a < L/64) and W_{RL}^{b i} (b < 64), and
 // W_{RL}^{p i} = W_{RL/64}^{(p>>6) i} W_{RL}^{(p&63) i}.  Layout per stage: the
-// direct segment, or the a segment then the b segment, i - 1 fastest.
+// direct segment, or the a segment then the b segment, each i-major (p fastest,
+// so the lanes of a wave read consecutive LDS words).
 static void split_values(int logM, std::vector<double>& re, std::vector<double>& im, std::vector<size_t>& segs) {
     const StagePlan p = stage_plan(logM);
     re.clear();
@@ -209,15 +210,15 @@ static void split_values(int logM, std::vector<double>& re, std::vector<double>&
         const uint64_t R = (uint64_t)p.rad[st], L = (uint64_t)p.L[st], RL = R * L;
         if (L <= 64) {
             segs.push_back(re.size());
-            for (uint64_t j = 0; j < L; ++j)
-                for (uint64_t i = 1; i < R; ++i) put(i * j, RL);
+            for (uint64_t i = 1; i < R; ++i)
+                for (uint64_t j = 0; j < L; ++j) put(i * j, RL);
         } else {
             segs.push_back(re.size());
-            for (uint64_t a = 0; a < L / 64; ++a)
-                for (uint64_t i = 1; i < R; ++i) put(i * a, RL / 64);
+            for (uint64_t i = 1; i < R; ++i)
+                for (uint64_t a = 0; a < L / 64; ++a) put(i * a, RL / 64);
             segs.push_back(re.size());
-            for (uint64_t b = 0; b < 64; ++b)
-                for (uint64_t i = 1; i < R; ++i) put(i * b, RL);
+            for (uint64_t i = 1; i < R; ++i)
+                for (uint64_t b = 0; b < 64; ++b) put(i * b, RL);
         }
     }
     if (!segs.empty() && segs[0] == 0) segs.erase(segs.begin());
@@ -390,16 +391,18 @@ __device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, in
 #pragma unroll
                 for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
             } else if constexpr (lgL <= 6) {
-                const C* t = tw + SplitPlan<LGM>::off(S) + pq * (RP - 1);
+                const C* t = tw + SplitPlan<LGM>::off(S) + pq;
 #pragma unroll
-                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[i - 1]);
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], t[(i - 1) << lgL]);
             } else {
+                constexpr int LA = (1 << lgL) / 64;
                 constexpr int OA = SplitPlan<LGM>::off(S);
-                constexpr int OB = OA + ((1 << lgL) / 64) * (RP - 1);
-                const C* ta = tw + OA + (pq >> 6) * (RP - 1);
-                const C* tb = tw + OB + (pq & 63) * (RP - 1);
+                constexpr int OB = OA + LA * (RP - 1);
+                const C* ta = tw + OA + (pq >> 6);
+                const C* tb = tw + OB + (pq & 63);
 #pragma unroll
-                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(ta[i - 1], tb[i - 1]));
+                for (int i = 1; i < RP; ++i)
+                    x[k * RP + i] = cmul(x[k * RP + i], cmul(ta[(i - 1) * LA], tb[(i - 1) * 64]));
             }
         }
         dft_r<RP, false>(x + k * RP);

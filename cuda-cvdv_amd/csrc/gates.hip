@@ -646,12 +646,17 @@ static void bs_small(CVDVContext* ctx, int r1, int r2, double th, int c = -1, in
     // gates.cu:614-636 (and 640-666 with a control)
     if (std::fabs(th) < 1e-15) return;
     const double cq = -std::tan(0.25 * th), cp = -std::sin(0.5 * th);
+    // The two transforms act on different registers and commute.  The first one
+    // applies the pending q1*q2 phase; the register with the lower physical
+    // field goes first, since its rows are contiguous, where the phase reduces
+    // to one affine term per row instead of a per-element evaluation.
+    const int fa = ctx->pshift[r1] <= ctx->pshift[r2] ? r1 : r2, fb = fa == r1 ? r2 : r1;
     phase_bilin(ctx, r1, r2, cq, c, q);
-    ctx_ft(ctx, r1, false);
-    ctx_ft(ctx, r2, false);
+    ctx_ft(ctx, fa, false);
+    ctx_ft(ctx, fb, false);
     phase_bilin(ctx, r1, r2, cp, c, q);
-    ctx_ft(ctx, r1, true);
-    ctx_ft(ctx, r2, true);
+    ctx_ft(ctx, fa, true);
+    ctx_ft(ctx, fb, true);
     phase_bilin(ctx, r1, r2, cq, c, q);
 }
 

@@ -526,8 +526,9 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
 // Pending phases, the four-step twiddle W_N^{j1 k2} of a second pass (row j1 =
 // the transformed index, k2 = (g >> tw4Shift) mod 2^tw4HiBits; TW4: the
 // instance may carry one) and the input conjugation of an inverse on the loaded
-// inputs.
-template <int RP, int E, int LGM, int LGC, bool TW4, typename C>
+// inputs.  ANYTAB: tables may need the per-element flat index (else the host
+// has checked that every table is read by row, table_by_row()).
+template <int RP, int E, int LGM, int LGC, bool TW4, bool ANYTAB, typename C>
 __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
                                            const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt) {
     constexpr int NB = E / RP;
@@ -586,31 +587,53 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         // first table is applied in straight-line code, so its loads issue with
         // the tile's input loads and share their wait (a loop that touches x
         // would wait for x's loads before issuing its first table load).
-        if (pre.ntab > 0) {
-            const DiagTab T = P.tab[0];
+        // A table whose field misses the tile's column and row-block bits, and
+        // holds all or none of its row bits, is read at base + (row << s) with a
+        // per-tile base and conjugation: 32-bit indices instead of each element's
+        // 64-bit flat index.
+        const int rowLo = d.lgInR, rowHi = d.lgInR + LGM;
+        auto hits = [](int sh, int nb, int lo, int n) { return n > 0 && sh < lo + n && lo < sh + nb; };
+        auto apply_table = [&](const DiagTab& T) {
             constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
+            const bool rowIn = T.sh <= rowLo && rowHi <= T.sh + T.nb;
+            const bool ctrlVar = T.ctrl >= 0 && (T.ctrl < LGC || (T.ctrl >= rowLo && T.ctrl < rowHi) ||
+                                                 (d.lgHb > 0 && T.ctrl >= d.lgInH2 && T.ctrl < d.lgInH2 + d.lgHb));
+            const bool fast = !hits(T.sh, T.nb, 0, LGC) && !hits(T.sh, T.nb, d.lgInH2, d.lgHb) &&
+                              (rowIn || !hits(T.sh, T.nb, rowLo, LGM)) && !ctrlVar;
+            if (!ANYTAB || fast) {
+                const C* tb = (const C*)pre.arena + T.off + (uint32_t)((inBase >> T.sh) & ((1ull << T.nb) - 1ull));
+                const int s = rowIn ? rowLo - T.sh : 31;  // 31: the row does not index the table
+                const bool cj = T.ctrl >= 0 && ((inBase >> T.ctrl) & 1ull);
 #pragma unroll
-            for (int j0 = 0; j0 < E; j0 += CH) {
-                C e[CH];
+                for (int j0 = 0; j0 < E; j0 += CH) {
+                    C e[CH];
 #pragma unroll
-                for (int j = 0; j < CH; ++j) e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+                    for (int j = 0; j < CH; ++j) {
+                        const BflyPos p = bfly_pos<RP, LGM>(tt + ((j0 + j) / RP) * nt, LGC);
+                        const uint32_t row = (uint32_t)(p.q + (((j0 + j) % RP) << lgMq));
+                        e[j] = tb[s < 31 ? (row << s) : 0u];
+                        if (cj) e[j].y = -e[j].y;
+                    }
 #pragma unroll
-                for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
+                    for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
+                }
+            } else if constexpr (ANYTAB) {
+#pragma unroll
+                for (int j0 = 0; j0 < E; j0 += CH) {
+                    C e[CH];
+#pragma unroll
+                    for (int j = 0; j < CH; ++j)
+                        e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+#pragma unroll
+                    for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
+                }
             }
-        }
+        };
+        if (pre.ntab > 0) apply_table(P.tab[0]);
         for (int t = 1; t < pre.ntab; ++t) {
             // opaque: the element indices are recomputed per table, not held
             asm volatile("" : "+v"(tt));
-            const DiagTab T = P.tab[t];
-            constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;
-#pragma unroll
-            for (int j0 = 0; j0 < E; j0 += CH) {
-                C e[CH];
-#pragma unroll
-                for (int j = 0; j < CH; ++j) e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
-#pragma unroll
-                for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
-            }
+            apply_table(P.tab[t]);
         }
         if (pre.ngen > 0) {
             asm volatile("" : "+v"(tt));
@@ -754,7 +777,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     auto prephase = [&](C* x, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        stage0_pre<R0, E, LGM, LGC, !PF>(x, d, inBase, P, pre, tw4, ti, nt);
+        stage0_pre<R0, E, LGM, LGC, !PF, !PF>(x, d, inBase, P, pre, tw4, ti, nt);
     };
     // Transform the phased tile in registers (the last stage's outputs stay in x).
     auto stages = [&](C* x, int ti, const C* twt) {
@@ -963,6 +986,20 @@ constexpr bool tile_instance() {
     return LGM + LGC <= (c64 ? 13 : 12);
 }
 
+// Every table of the program is read by row in a one-row tile over the field
+// [fs, fs + fn): its field holds the whole row field or none of it, and its
+// control bit lies outside (the persistent instances compile only that path).
+static bool table_by_row(const FastDiag& pre, int fs, int fn) {
+    for (int i = 0; i < pre.ntab; ++i) {
+        const DiagTab& T = pre.tab[i];
+        const bool in = T.sh <= fs && fs + fn <= T.sh + T.nb;
+        const bool out = T.sh >= fs + fn || fs >= T.sh + T.nb;
+        if (!in && !out) return false;
+        if (T.ctrl >= fs && T.ctrl < fs + fn) return false;
+    }
+    return true;
+}
+
 template <typename R, int LGM, int LGC>
 static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
                         const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
@@ -982,20 +1019,22 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
         // and keep their (split) twiddles in LDS
         // (the second pass of a four-step split, which carries the four-step
-        // twiddle, runs one tile per workgroup)
+        // twiddle, and a program with a table not read by row run one tile per
+        // workgroup)
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
-        if (PF && !d.tw4) {
+        const bool pf = PF && !d.tw4 && (!d.usePre || table_by_row(pre, d.lgInR, LGM));
+        if (pf) {
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
         }
         if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
             if (tile / 8 > 1024) {
-                if (!d.tw4) go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
+                if (pf) go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);
                 else go(k_fft_tile<R, LGM, LGC, 16, false>, 16, false);
                 return;
             }
         }
-        if (PF && !d.tw4) go(k_fft_tile<R, LGM, LGC, 8, PF>, 8, PF);
+        if (pf) go(k_fft_tile<R, LGM, LGC, 8, PF>, 8, PF);
         else go(k_fft_tile<R, LGM, LGC, 8, false>, 8, false);
     } else {
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile shape without a kernel instance"};

@@ -642,15 +642,20 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             if (LGC == 0 && d.lgHb == 0) gf = fold_generic(P, pre, inBase, d.lgInR, LGM);
             if (gf.ok) {
                 using R = decltype(C{}.x);
+                // A butterfly's rows are q + i M/RP, so its phases are e^{2 pi i (a m0 + b)}
+                // times the tile-wide steps e^{2 pi i a 2 i M/RP}: one sin/cos per
+                // butterfly, and RP-1 per tile (uniform across the wave).
+                C st[RP];
+#pragma unroll
+                for (int i = 1; i < RP; ++i) st[i] = unit_rev_fast(cvdv_frac(gf.a * (double)(2 * (i << lgMq))), R(0));
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+                    const double m0 = (double)(2 * p.q + 1 - (1 << LGM));
+                    const C e0 = unit_rev_fast(cvdv_frac(fma(gf.a, m0, gf.b)), R(0));
+                    x[k * RP] = cmul(x[k * RP], e0);
 #pragma unroll
-                    for (int i = 0; i < RP; ++i) {
-                        const int q = p.q + (i << lgMq);
-                        const double m = (double)(2 * q + 1 - (1 << LGM));
-                        x[k * RP + i] = cmul(x[k * RP + i], unit_rev_fast(cvdv_frac(fma(gf.a, m, gf.b)), R(0)));
-                    }
+                    for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
                 }
             } else {
 #pragma unroll

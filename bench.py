@@ -548,6 +548,68 @@ def copy_probe(reps=10):
     return {"bytes_per_copy": 2 * A * 8, "ms": ms, "gbps": 2 * A * 8 / (ms * 1e-3) / 1e9}
 
 
+def dft_gemm_probe(reps=5):
+    """DFT-as-GEMM against the register FFT on one 2^28-amplitude complex64 state.
+
+    For N in {64, 256, 1024}: the DFT of the leading axis of an (N, 2^28/N) view as
+    one complex64 GEMM F_N @ X (rocBLAS/hipBLASLt through torch), the real bf16
+    GEMM of the same shape that one term of a split-bf16 scheme needs (three such
+    terms give FP32-like accuracy), and the library's FFT of an N-point register
+    of a [28 - log2 N, log2 N] state (one pass, the register contiguous)."""
+    import torch
+    from hipcvdv import HipCvdv, SeparableState
+
+    A = 1 << sum(REGS)
+    out = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    for lg in (6, 8, 10):
+        N = 1 << lg
+        B = A // N
+        j = torch.arange(N, device="cuda", dtype=torch.float64)
+        F = torch.exp(-2j * math.pi * torch.outer(j, j) / N).to(torch.complex64) / math.sqrt(N)
+        X = torch.randn(N, B, device="cuda", dtype=torch.complex64)
+        Y = torch.empty_like(X)
+        ms_c64 = timed(lambda: torch.matmul(F, X, out=Y))
+        del X, Y
+        torch.cuda.empty_cache()
+        Fr = torch.randn(2 * N, 2 * N, device="cuda", dtype=torch.bfloat16)
+        Xr = torch.randn(2 * N, B, device="cuda", dtype=torch.bfloat16)
+        ms_bf16 = timed(lambda: torch.matmul(Fr, Xr))
+        del Fr, Xr
+        torch.cuda.empty_cache()
+        regs = [sum(REGS) - lg, lg]
+        h = HipCvdv(regs, precision="mixed")
+        sep = SeparableState(regs)
+        sep.setCoherent(0, 0.5)
+        sep.setCoherent(1, 0.3)
+        h.initStateVector(sep)
+        h.ftQ2P(1)
+        h.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            h.ftQ2P(1)
+        h.synchronize()
+        ms_fft = (time.perf_counter() - t0) / reps * 1e3  # the context's own stream
+        del h
+        torch.cuda.empty_cache()
+        out[f"N={N}"] = {"gemm_c64_ms": ms_c64, "gemm_bf16_one_term_ms": ms_bf16,
+                         "gemm_flops": 8.0 * N * N * B, "fft_pass_ms": ms_fft}
+        print(f"[bench] dft-gemm N={N}: c64 GEMM {ms_c64:.2f} ms, bf16 GEMM (one of 3 split terms) "
+              f"{ms_bf16:.2f} ms, register FFT {ms_fft:.3f} ms", file=sys.stderr, flush=True)
+    return out
+
+
 def pmc_traffic():
     """HBM bytes per FFT launch from the committed counter summary (FETCH_SIZE and
     WRITE_SIZE, each scaled by the same counter on a plain copy of known size)."""
@@ -579,12 +641,17 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the phase-space readout workloads (C1, C5)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the plain-copy bandwidth probe (counter calibration runs)")
+    ap.add_argument("--dft-gemm", action="store_true",
+                    help="only the DFT-as-GEMM (MFMA) against register-FFT comparison")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: runs on one GPU only; the state vector is not sharded across GPUs.",
               file=sys.stderr)
         sys.exit(2)
 
+    if args.dft_gemm:
+        print(json.dumps({"dft_gemm": dft_gemm_probe()}))
+        return
     if args.copy_only:
         print(json.dumps({"copy": copy_probe()}))
         return

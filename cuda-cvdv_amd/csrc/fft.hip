@@ -7,16 +7,17 @@
 // C consecutive lo columns x M rows x Hb consecutive hi blocks in LDS, runs a
 // Stockham FFT there (radix 8/4/2 butterflies in registers, per-stage twiddle
 // tables generated in FP64), and writes the tile back:
-//   * one HBM pass when the tile fits in LDS: C*M elements with C >= 16 (c64)
-//     or 8 (c128) columns for a strided register, or a whole contiguous
-//     register of up to 2^14 (c64) / 2^13 (c128) points in one workgroup;
+//   * one HBM pass when the tile fits in LDS: C*M elements with C = 32 (c64)
+//     or 16 (c128) columns (256 bytes per row) for a strided register, or a
+//     whole contiguous register of up to 2^14 (c64) / 2^13 (c128) points in one
+//     workgroup (persistent and prefetching for c64 rows of 2^13 and 2^14);
 //   * otherwise the four-step split N = N1 * N2 (j = j1 + N1 j2):
-//       pass 1 (in place):  FFT over j2, times W_N^{j1 k2};
-//       pass 2 (out of place, transposing): FFT over j1, written at k2 + N2 k1.
+//       pass 1 (in place):  FFT over j2;
+//       pass 2 (out of place, transposing): times W_N^{j1 k2} on load, FFT
+//       over j1, written at k2 + N2 k1.
 // Pending diagonal phases are applied (compiled, see diag_fast.h) as elements
 // are loaded by the first pass and 1/sqrt(N) as they are stored by the last.
 // Every tile dimension is a power of two: indices use shifts and masks only.
-// complex64 tiles move 16 bytes per lane along the contiguous dimension.
 
 #include <algorithm>
 #include <cmath>
@@ -960,7 +961,7 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     };
     d.vecOut = vec_ok(d.rowFastOut, g.outR, g.outH2);
     // last stage writes rows q + m*M/R of column c: coalesced when a full
-    // column group is one 128-byte run or when rows follow the columns directly
+    // column group is one full-width run or when rows follow the columns directly
     d.directOut = (C == W || g.outR == C) ? 1 : 0;
     return true;
 }
@@ -981,7 +982,7 @@ static uint64_t resident_blocks(const void* kfn, int threads, size_t lds) {
 }
 
 // Kernel instances exist for the tile shapes fit_pass can produce: C = 2^LGC
-// columns (up to 128 bytes per row) with C*M within 64 KiB of LDS, or one whole
+// columns (up to 256 bytes per row) with C*M within 64 KiB of LDS, or one whole
 // contiguous row (C = 1) of up to 2^14 complex64 / 2^13 complex128 points.
 template <typename R, int LGM, int LGC>
 constexpr bool tile_instance() {

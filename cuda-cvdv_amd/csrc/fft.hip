@@ -748,13 +748,23 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     const int tile = 1 << (LGM + lgC + lgHb);
 
     if (d.usePre) {
-        // constant indices: the argument is read in place
+        // constant indices: the argument is read in place (one entry per thread,
+        // or all by thread 0 in a tile with fewer threads than entries)
+        if (nt >= CVDV_FAST_TABS + CVDV_FAST_GEN) {
 #pragma unroll
-        for (int i = 0; i < CVDV_FAST_TABS; ++i)
-            if (tid == i && i < pre.ntab) P.tab[i] = pre.tab[i];
+            for (int i = 0; i < CVDV_FAST_TABS; ++i)
+                if (tid == i && i < pre.ntab) P.tab[i] = pre.tab[i];
 #pragma unroll
-        for (int i = 0; i < CVDV_FAST_GEN; ++i)
-            if (tid == CVDV_FAST_TABS + i && i < pre.ngen) P.gen[i] = pre.gen[i];
+            for (int i = 0; i < CVDV_FAST_GEN; ++i)
+                if (tid == CVDV_FAST_TABS + i && i < pre.ngen) P.gen[i] = pre.gen[i];
+        } else if (tid == 0) {
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_TABS; ++i)
+                if (i < pre.ntab) P.tab[i] = pre.tab[i];
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_GEN; ++i)
+                if (i < pre.ngen) P.gen[i] = pre.gen[i];
+        }
         __syncthreads();
     }
 

@@ -112,7 +112,10 @@ def _random_ops(rng, regs, n):
                                        ([1, 4, 11], 17), ([1, 2, 13], 18),
                                        # a contiguous register above one tile: two-pass FFT whose
                                        # second pass has one column and carries the twiddle
-                                       ([1, 15], 19)])
+                                       ([1, 15], 19),
+                                       # 32 amplitudes: FFT tiles of 4 threads, fewer than the
+                                       # phase program's entries (staged by one thread)
+                                       ([1, 2, 2], 20)])
 def test_random_circuit_parity(regs, seed, prec):
     print(f"[gpu] random {regs} seed {seed} {prec}", flush=True)
     rng = np.random.default_rng(seed)

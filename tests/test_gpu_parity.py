@@ -115,7 +115,10 @@ def _random_ops(rng, regs, n):
                                        ([1, 15], 19),
                                        # 32 amplitudes: FFT tiles of 4 threads, fewer than the
                                        # phase program's entries (staged by one thread)
-                                       ([1, 2, 2], 20)])
+                                       ([1, 2, 2], 20),
+                                       # two-pass FFTs of a 2^16 contiguous register (narrowed
+                                       # pass-1 tiles) and of a strided 2^13 register
+                                       ([1, 16], 21), ([1, 13, 2], 22)])
 def test_random_circuit_parity(regs, seed, prec):
     print(f"[gpu] random {regs} seed {seed} {prec}", flush=True)
     rng = np.random.default_rng(seed)
@@ -126,7 +129,11 @@ def test_random_circuit_parity(regs, seed, prec):
     err = np.max(np.abs(hs - os_)) / np.max(np.abs(os_))
     fl = fidelity_loss(hs, os_)
     print(f"    rel max amp err {err:.3e}, fidelity loss {fl:.3e}")
-    assert err <= (1e-11 if prec == "complex128" else 5e-5)
+    # complex128: the phase angles grow like x^3 (phaseCubic) with |x| up to
+    # sqrt(pi N / 2), ~1e5 rad at N = 2^16, so two FP64 evaluations of the same
+    # angle differ by ~1e-11 rad there; the bar scales with the largest register
+    c128_bar = 1e-11 if max(regs) <= 13 else 1e-10
+    assert err <= (c128_bar if prec == "complex128" else 5e-5)
     assert fl <= (1e-12 if prec == "complex128" else 1e-6)
     # complex64 amplitudes: probabilities carry the FP32 norm drift of ~200 passes
     mtol = read_tol(prec) if prec == "complex128" else 2e-5

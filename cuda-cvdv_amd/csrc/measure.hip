@@ -224,17 +224,22 @@ __device__ void block_reduce_store(CSum v, CSum* out) {
 template <typename C>
 __global__ void __launch_bounds__(256) k_dot_sep(const C* __restrict__ st, uint64_t total, DiagProg p, int usep,
                                                  SepDot sd, CSum* out) {
+    // the register loop indexes the descriptor at run time: staged in LDS, not
+    // copied to per-lane scratch memory
+    __shared__ SepDot s;
+    if (threadIdx.x == 0) s = sd;
+    __syncthreads();
     CSum acc{0, 0, 0, 0};
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         C v = st[g];
         if (usep) v = apply_diag(p, g, v);
         double2 t = make_double2(1.0, 0.0);
-        for (int r = 0; r < sd.nreg; ++r) {
-            const uint64_t m = (1ull << sd.nb[r]) - 1ull;
-            uint64_t j = (g >> sd.sh[r]) & m;
-            if (sd.refl[r]) j = m - j;
-            t = cmul(t, sd.arr[r][j]);
+        for (int r = 0; r < s.nreg; ++r) {
+            const uint64_t m = (1ull << s.nb[r]) - 1ull;
+            uint64_t j = (g >> s.sh[r]) & m;
+            if (s.refl[r]) j = m - j;
+            t = cmul(t, s.arr[r][j]);
         }
         const double2 pr = cmulc(t, make_double2((double)v.x, (double)v.y));  // conj(psi) * t
         ksum_add(acc.sx, acc.cx, pr.x);
@@ -243,17 +248,33 @@ __global__ void __launch_bounds__(256) k_dot_sep(const C* __restrict__ st, uint6
     block_reduce_store(acc, out);
 }
 
+// apply_diag for a program in LDS: a plain loop over its terms (the unrolled
+// form would hold every term of both programs in registers across the loop).
+template <typename C>
+__device__ __forceinline__ C apply_diag_lds(const DiagProg& p, uint64_t g, C v) {
+    using R = decltype(v.x);
+    double r = p.grev;
+#pragma unroll 1
+    for (int i = 0; i < p.n; ++i) r += cvdv_frac(diag_term_rev(p.t[i], g));
+    return cmul(v, unit_rev(cvdv_frac(r), R(0)));
+}
+
 // sum_g conj(a_g) b_g   (measure.cuh:227-252)
 template <typename CA, typename CB>
 __global__ void __launch_bounds__(256) k_dot_states(const CA* __restrict__ a, const CB* __restrict__ b, uint64_t total,
                                                     DiagProg pa, int usea, DiagProg pb, int useb, CSum* out) {
+    // two programs held in registers would spill: they are read from LDS
+    __shared__ DiagProg spa, spb;
+    if (threadIdx.x == 0) spa = pa;
+    if (threadIdx.x == 1) spb = pb;
+    __syncthreads();
     CSum acc{0, 0, 0, 0};
     for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
          g += (uint64_t)gridDim.x * blockDim.x) {
         CA va = a[g];
         CB vb = b[g];
-        if (usea) va = apply_diag(pa, g, va);
-        if (useb) vb = apply_diag(pb, g, vb);
+        if (usea) va = apply_diag_lds(spa, g, va);
+        if (useb) vb = apply_diag_lds(spb, g, vb);
         const double2 pr = cmulc(make_double2((double)vb.x, (double)vb.y),
                                  make_double2((double)va.x, (double)va.y));
         ksum_add(acc.sx, acc.cx, pr.x);

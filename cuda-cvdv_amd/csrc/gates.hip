@@ -19,21 +19,40 @@ static const double PI = 3.14159265358979323846;
 static const double SQRT2 = 1.41421356237309504880;
 
 // ---------------------------------------------------------------- kernels
+// The element-wise and pair kernels give each thread this many elements (pairs)
+// per iteration, their loads issued together.
+#define PAIRS_PER_THREAD 4
+
 template <typename C>
 __global__ void k_diag(C* __restrict__ s, uint64_t total, FastDiag f) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x)
-        s[g] = apply_fast(f, g, s[g]);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < total; g0 += PAIRS_PER_THREAD * stride) {
+        C v[PAIRS_PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k)
+            if (g0 + k * stride < total) v[k] = s[g0 + k * stride];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k)
+            if (g0 + k * stride < total) s[g0 + k * stride] = apply_fast(f, g0 + k * stride, v[k]);
+    }
 }
 
 // A whole program evaluated per element in FP64 (programs beyond the compiled
-// capacities; rare).
+// capacities; rare): the angle in FP64, then unit_rev_fast's sin/cos.
 template <typename C>
 __global__ void k_diag_prog(C* __restrict__ s, uint64_t total, DiagProg p) {
     using R = decltype(C{}.x);
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x)
-        s[g] = cmul(s[g], unit_rev(diag_rev(p, g), R(0)));
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t g0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g0 < total; g0 += PAIRS_PER_THREAD * stride) {
+        C v[PAIRS_PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k)
+            if (g0 + k * stride < total) v[k] = s[g0 + k * stride];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k)
+            if (g0 + k * stride < total)
+                s[g0 + k * stride] = cmul(v[k], unit_rev_fast(diag_rev(p, g0 + k * stride), R(0)));
+    }
 }
 
 struct TabMake {
@@ -79,20 +98,38 @@ template <typename C>
 __global__ void k_qubit(C* __restrict__ s, uint64_t npairs, int bit, double2 m00, double2 m01,
                         double2 m10, double2 m11, FastDiag p, int usep) {
     const uint64_t lowmask = (1ull << bit) - 1ull;
-    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
-         pid += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);
-        const uint64_t i1 = i0 | (1ull << bit);
-        C a = s[i0], b = s[i1];
-        if (usep) {
-            a = apply_fast(p, i0, a);
-            b = apply_fast(p, i1, b);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    // PAIRS_PER_THREAD pairs per iteration, all loaded before any is used: a
+    // single pair per thread leaves too few bytes in flight to cover HBM latency
+    for (uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p0 < npairs;
+         p0 += PAIRS_PER_THREAD * stride) {
+        C a[PAIRS_PER_THREAD], b[PAIRS_PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k) {
+            const uint64_t pid = p0 + k * stride;
+            if (pid < npairs) {
+                const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);
+                a[k] = s[i0];
+                b[k] = s[i0 | (1ull << bit)];
+            }
         }
-        const double2 ad = make_double2((double)a.x, (double)a.y);
-        const double2 bd = make_double2((double)b.x, (double)b.y);
-        const double2 t00 = cmul(m00, ad), t01 = cmul(m01, bd), t10 = cmul(m10, ad), t11 = cmul(m11, bd);
-        s[i0] = cmk<C>(t00.x + t01.x, t00.y + t01.y);
-        s[i1] = cmk<C>(t10.x + t11.x, t10.y + t11.y);
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k) {
+            const uint64_t pid = p0 + k * stride;
+            if (pid >= npairs) continue;
+            const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);
+            const uint64_t i1 = i0 | (1ull << bit);
+            C va = a[k], vb = b[k];
+            if (usep) {
+                va = apply_fast(p, i0, va);
+                vb = apply_fast(p, i1, vb);
+            }
+            const double2 ad = make_double2((double)va.x, (double)va.y);
+            const double2 bd = make_double2((double)vb.x, (double)vb.y);
+            const double2 t00 = cmul(m00, ad), t01 = cmul(m01, bd), t10 = cmul(m10, ad), t11 = cmul(m11, bd);
+            s[i0] = cmk<C>(t00.x + t01.x, t00.y + t01.y);
+            s[i1] = cmk<C>(t10.x + t11.x, t10.y + t11.y);
+        }
     }
 }
 
@@ -105,23 +142,39 @@ __global__ void k_cparity(C* __restrict__ s, uint64_t npairs, int sh, int nb, in
     const int top = sh + nb - 1;
     const uint64_t lowmask = (1ull << top) - 1ull;
     const uint64_t fmask = ((1ull << nb) - 1ull) << sh;
-    for (uint64_t pid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; pid < npairs;
-         pid += (uint64_t)gridDim.x * blockDim.x) {
-        const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);  // top field bit 0
-        const uint64_t i1 = i0 ^ fmask;
-        C a = s[i0], b = s[i1];
-        if (usep) {
-            a = apply_fast(p, i0, a);
-            b = apply_fast(p, i1, b);
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t p0 = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; p0 < npairs;
+         p0 += PAIRS_PER_THREAD * stride) {
+        C a[PAIRS_PER_THREAD], b[PAIRS_PER_THREAD];
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k) {
+            const uint64_t pid = p0 + k * stride;
+            if (pid < npairs) {
+                const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);  // top field bit 0
+                a[k] = s[i0];
+                b[k] = s[i0 ^ fmask];
+            }
         }
-        const uint64_t lower = tgtRefl ? i1 : i0;
-        const int cb = (int)((lower >> ctrlPos) & 1ull) ^ ctrlFlip;
-        if (cb) {
-            s[i0] = b;
-            s[i1] = a;
-        } else if (usep) {
-            s[i0] = a;
-            s[i1] = b;
+#pragma unroll
+        for (int k = 0; k < PAIRS_PER_THREAD; ++k) {
+            const uint64_t pid = p0 + k * stride;
+            if (pid >= npairs) continue;
+            const uint64_t i0 = ((pid & ~lowmask) << 1) | (pid & lowmask);
+            const uint64_t i1 = i0 ^ fmask;
+            C va = a[k], vb = b[k];
+            if (usep) {
+                va = apply_fast(p, i0, va);
+                vb = apply_fast(p, i1, vb);
+            }
+            const uint64_t lower = tgtRefl ? i1 : i0;
+            const int cb = (int)((lower >> ctrlPos) & 1ull) ^ ctrlFlip;
+            if (cb) {
+                s[i0] = vb;
+                s[i1] = va;
+            } else if (usep) {
+                s[i0] = va;
+                s[i1] = vb;
+            }
         }
     }
 }
@@ -151,10 +204,10 @@ static void diag_prog_pass(CVDVContext* ctx, const DiagProg& p) {
     const uint64_t n = ctx->totalSize;
     const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
-        hipLaunchKernelGGL(k_diag_prog<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diag_prog<double2>, dim3(grid_for((n + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), n, p);
     else
-        hipLaunchKernelGGL(k_diag_prog<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diag_prog<float2>, dim3(grid_for((n + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (float2*)ctx->state(), n, p);
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
@@ -315,10 +368,10 @@ void launch_diag(CVDVContext* ctx, const DiagProg& pp) {
     const FastDiag p = compile_diag(ctx, pp, act);
     const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
-        hipLaunchKernelGGL(k_diag<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diag<double2>, dim3(grid_for((n + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), n, p);
     else
-        hipLaunchKernelGGL(k_diag<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_diag<float2>, dim3(grid_for((n + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (float2*)ctx->state(), n, p);
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
@@ -475,10 +528,10 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
     const double2 m10 = make_double2(mm[4], mm[5]), m11 = make_double2(mm[6], mm[7]);
     const int pid = prof_begin(ctx, KK_QUBIT);
     if (ctx->prec == CVDV_C128)
-        hipLaunchKernelGGL(k_qubit<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_qubit<double2>, dim3(grid_for((np + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), np, bit, m00, m01, m10, m11, p, usep);
     else
-        hipLaunchKernelGGL(k_qubit<float2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_qubit<float2>, dim3(grid_for((np + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (float2*)ctx->state(), np, bit, m00, m01, m10, m11, p, usep);
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
@@ -541,11 +594,11 @@ static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
     const uint64_t np = ctx->totalSize / 2;
     const int pid = prof_begin(ctx, KK_CPARITY);
     if (ctx->prec == CVDV_C128)
-        hipLaunchKernelGGL(k_cparity<double2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_cparity<double2>, dim3(grid_for((np + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (double2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
                            ctx->refl[tgt], p, usep);
     else
-        hipLaunchKernelGGL(k_cparity<float2>, dim3(grid_for(np)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL(k_cparity<float2>, dim3(grid_for((np + PAIRS_PER_THREAD - 1) / PAIRS_PER_THREAD)), dim3(256), 0, ctx->stream,
                            (float2*)ctx->state(), np, ctx->pshift[tgt], nb, cpos, ctx->refl[creg],
                            ctx->refl[tgt], p, usep);
     HIPCHK(hipGetLastError());

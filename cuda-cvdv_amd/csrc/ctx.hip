@@ -152,14 +152,43 @@ struct SepArgs {
     int nreg;
 };
 
-// psi[g] = prod_r arr_r[local_r(g)], registers multiplied in order 0..R-1.
+// psi[g] = prod_r arr_r[local_r(g)].  The state is cut into chunks of 2^k
+// amplitudes (k <= INIT_LO_BITS).  Registers wholly below bit k form a product
+// table in LDS, built once per workgroup; registers wholly above it give one
+// factor per chunk; the register (at most one) that straddles bit k is read per
+// amplitude.  An amplitude then costs one LDS read, at most one gather from a
+// register array (cache-resident), two FP64 complex products and one store,
+// instead of one gather and one product per register.
+#define INIT_LO_BITS 11
 template <typename C>
-__global__ void k_init_product(C* __restrict__ out, uint64_t total, SepArgs a) {
-    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < total;
-         g += (uint64_t)gridDim.x * blockDim.x) {
-        double2 p = a.arr[0][(g >> a.sh[0]) & ((1ull << a.nb[0]) - 1)];
-        for (int r = 1; r < a.nreg; ++r) p = cmul(p, a.arr[r][(g >> a.sh[r]) & ((1ull << a.nb[r]) - 1)]);
-        out[g] = cmk<C>(p.x, p.y);
+__global__ void __launch_bounds__(256) k_init_product(C* __restrict__ out, uint64_t nchunks, int k,
+                                                      SepArgs a) {
+    __shared__ double2 lo[1 << INIT_LO_BITS];
+    const int L = 1 << k;
+    int s = -1;
+    for (int r = 0; r < a.nreg; ++r)
+        if (a.sh[r] < k && a.sh[r] + a.nb[r] > k) s = r;
+    for (int l = threadIdx.x; l < L; l += blockDim.x) {
+        double2 p = make_double2(1.0, 0.0);
+        for (int r = 0; r < a.nreg; ++r)
+            if (a.sh[r] + a.nb[r] <= k) p = cmul(p, a.arr[r][(l >> a.sh[r]) & ((1 << a.nb[r]) - 1)]);
+        lo[l] = p;
+    }
+    __syncthreads();
+    const double2* sa = s >= 0 ? a.arr[s] : nullptr;
+    const int ssh = s >= 0 ? a.sh[s] : 0;
+    const uint64_t smask = s >= 0 ? (1ull << a.nb[s]) - 1 : 0;
+    for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
+        const uint64_t g0 = c << k;
+        double2 hp = make_double2(1.0, 0.0);
+        for (int r = 0; r < a.nreg; ++r)
+            if (a.sh[r] >= k) hp = cmul(hp, a.arr[r][(g0 >> a.sh[r]) & ((1ull << a.nb[r]) - 1)]);
+#pragma unroll 4
+        for (int l = threadIdx.x; l < L; l += blockDim.x) {
+            double2 p = cmul(hp, lo[l]);
+            if (sa) p = cmul(p, sa[((g0 | (uint64_t)l) >> ssh) & smask]);
+            out[g0 + l] = cmk<C>(p.x, p.y);
+        }
     }
 }
 
@@ -437,13 +466,15 @@ void cvdvInitFromSeparable(CVDVContext* ctx, void** devicePtrs, int numReg) {
     // The register arrays were written on another stream (the caller's); the
     // copy below must see them, and they may be freed as soon as we return.
     HIPCHK(hipDeviceSynchronize());
-    const uint64_t n = ctx->totalSize;
+    const int k = std::min(INIT_LO_BITS, ctx->totalQbt);
+    const uint64_t nchunks = ctx->totalSize >> k;
+    const dim3 grid((unsigned)std::min<uint64_t>(nchunks, 2048));
     if (ctx->prec == CVDV_C128)
-        hipLaunchKernelGGL(k_init_product<double2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                           (double2*)ctx->state(), n, a);
+        hipLaunchKernelGGL(k_init_product<double2>, grid, dim3(256), 0, ctx->stream,
+                           (double2*)ctx->state(), nchunks, k, a);
     else
-        hipLaunchKernelGGL(k_init_product<float2>, dim3(grid_for(n)), dim3(256), 0, ctx->stream,
-                           (float2*)ctx->state(), n, a);
+        hipLaunchKernelGGL(k_init_product<float2>, grid, dim3(256), 0, ctx->stream,
+                           (float2*)ctx->state(), nchunks, k, a);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(ctx->stream));
     ctx->initialized = true;
@@ -574,7 +605,7 @@ void cvdvGraphBegin(CVDVContext* ctx) {
     (void)ctx->other();  // both buffers exist before the capture
     int maxNb = 0;
     for (int r = 0; r < ctx->numReg; ++r) maxNb = std::max(maxNb, ctx->nq[r]);
-    arena_reserve(ctx, ((size_t)CVDV_FAST_TABS * ((size_t)1 << std::min(maxNb, 20)) + 1) * ctx->elemBytes());
+    arena_reserve(ctx, diag_arena_bound(maxNb, ctx->elemBytes()));
     fft_prepare(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int r = 0; r < ctx->numReg; ++r) {

@@ -2,31 +2,45 @@
 //
 // Before a kernel that loads the state runs, the part of the pending DiagProg
 // that does not commute with it is split into:
-//   * tables: every single-register polynomial term (and every Rz bit term)
-//     becomes a table of e^{2 pi i phase(j)} over that register's local index,
-//     evaluated in FP64 (sincospi) by k_make_tables and rounded once to the
-//     state's precision; a control bit conjugates the looked-up entry.  The
-//     global phase and, for an FFT, the 1/sqrt(N) normalisation are folded
-//     into an unconditional table in FP64, so no rounded constant factor is
-//     applied per element;
+//   * tables: single-register polynomial terms (and Rz bit terms) become
+//     tables of e^{2 pi i phase} over that register's local index, evaluated in
+//     FP64 (sincospi) by k_make_tables and rounded once to the state's
+//     precision.  Terms over the same register share one table, with a
+//     sub-table per combination of their control bits (choice bits); one-bit
+//     terms on different bits share a table over the combinations of those
+//     bits; a lone conditional term keeps one table whose entry its control
+//     bit conjugates.  An element reads one entry per table.  The global phase
+//     and, for an FFT, the 1/sqrt(N) normalisation are folded in FP64 into a
+//     table that every element reads unconjugated, so no rounded constant
+//     factor is applied per element;
 //   * generic terms: bilinear q1*q2 terms (and polynomial terms of registers
 //     too large to tabulate), evaluated per element as one FP64 angle with
 //     range reduction, then sin/cos;
-//   * a one-entry table carries the global phase and scale when no
-//     unconditional table exists.
-// Capacities are fixed (CVDV_FAST_TABS tables, CVDV_FAST_GEN generic terms); a
-// program that does not fit is applied by a separate FP64 pass first.
+//   * a one-entry table carries the global phase and scale when no table
+//     is read by every element.
+// Capacities are fixed (CVDV_FAST_TABS tables, CVDV_FAST_GEN generic terms).
+// Tables beyond them are applied by table passes of their own first; too many
+// generic terms, by one FP64 pass of the whole program.
 #pragma once
 
 #include "cvdv_internal.h"
 
 #define CVDV_FAST_TABS 6
 #define CVDV_FAST_GEN 4
+#define CVDV_TAB_SEL 4        // choice bits of one table
+#define CVDV_TAB_MERGE_BITS 16  // a table shared by several terms holds at most 2^16 entries
 
+// Entry of element g: off + local(g over [sh, sh+nb)) + (choice(g) << nb), where
+// bit j < nsel of the choice is bit ((sel >> 8j) & 63) of g; ctrl >= 0
+// conjugates the entry when that bit of g is set.
 struct DiagTab {
     int sh, nb, ctrl;
     uint32_t off;  // first entry in the table arena
+    int nsel;
+    uint32_t sel;  // choice bit positions, one per byte
 };
+
+__host__ __device__ inline int tab_sel(const DiagTab& t, int j) { return (int)((t.sel >> (8 * j)) & 63u); }
 
 struct FastDiag {
     int ntab, ngen;
@@ -50,9 +64,19 @@ __device__ __forceinline__ float2 unit_rev_fast(double f, float) {
 }
 __device__ __forceinline__ double2 unit_rev_fast(double f, double) { return unit_rev(f, 0.0); }
 
+__device__ __forceinline__ uint32_t tab_choice(const DiagTab& t, uint64_t g) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int j = 0; j < CVDV_TAB_SEL; ++j)
+        if (j < t.nsel) c |= (uint32_t)((g >> tab_sel(t, j)) & 1ull) << j;
+    return c << t.nb;
+}
+
 template <typename C>
 __device__ __forceinline__ C tab_entry(const void* arena, const DiagTab& t, uint64_t g) {
-    C e = ((const C*)arena)[t.off + ((g >> t.sh) & ((1ull << t.nb) - 1ull))];
+    uint32_t i = t.off + (uint32_t)((g >> t.sh) & ((1ull << t.nb) - 1ull));
+    if (t.nsel > 0) i += tab_choice(t, g);
+    C e = ((const C*)arena)[i];
     if (t.ctrl >= 0 && ((g >> t.ctrl) & 1ull)) e.y = -e.y;
     return e;
 }
@@ -88,5 +112,11 @@ __device__ __forceinline__ C apply_fast(const FastDiag& f, uint64_t g, C v) {
 // compiled program; `active` is false when the program is the identity.
 // scale != 1 is folded into an unconditional table when one exists
 // (*scaleFolded tells the caller whether it still has to apply it).
+// tileBits: index bits that vary within one tile of a consumer that reads its
+// tables through a per-tile base; no choice bit is placed on them, so such a
+// consumer computes the choice once per tile.
 FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale = 1.0,
-                      bool* scaleFolded = nullptr);
+                      bool* scaleFolded = nullptr, uint64_t tileBits = 0);
+
+// Largest table arena compile_diag can need for registers of at most maxNb qubits.
+size_t diag_arena_bound(int maxNb, size_t elemBytes);

@@ -594,15 +594,32 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         // 64-bit flat index.
         const int rowLo = d.lgInR, rowHi = d.lgInR + LGM;
         auto hits = [](int sh, int nb, int lo, int n) { return n > 0 && sh < lo + n && lo < sh + nb; };
-        auto apply_table = [&](const DiagTab& T) {
+        auto apply_table = [&](const DiagTab& TL) {
+            // the descriptor is read from LDS into vector registers; it is the
+            // same for every lane, so keep it in scalar registers
+            DiagTab T;
+            T.sh = __builtin_amdgcn_readfirstlane(TL.sh);
+            T.nb = __builtin_amdgcn_readfirstlane(TL.nb);
+            T.ctrl = __builtin_amdgcn_readfirstlane(TL.ctrl);
+            T.off = (uint32_t)__builtin_amdgcn_readfirstlane((int)TL.off);
+            T.nsel = __builtin_amdgcn_readfirstlane(TL.nsel);
+            T.sel = (uint32_t)__builtin_amdgcn_readfirstlane((int)TL.sel);
             constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
             const bool rowIn = T.sh <= rowLo && rowHi <= T.sh + T.nb;
-            const bool ctrlVar = T.ctrl >= 0 && (T.ctrl < LGC || (T.ctrl >= rowLo && T.ctrl < rowHi) ||
-                                                 (d.lgHb > 0 && T.ctrl >= d.lgInH2 && T.ctrl < d.lgInH2 + d.lgHb));
+            auto varies = [&](int b) {
+                return b >= 0 && (b < LGC || (b >= rowLo && b < rowHi) ||
+                                  (d.lgHb > 0 && b >= d.lgInH2 && b < d.lgInH2 + d.lgHb));
+            };
+            bool ctrlVar = varies(T.ctrl);
+#pragma unroll
+            for (int j = 0; j < CVDV_TAB_SEL; ++j)
+                if (j < T.nsel) ctrlVar = ctrlVar || varies(tab_sel(T, j));
             const bool fast = !hits(T.sh, T.nb, 0, LGC) && !hits(T.sh, T.nb, d.lgInH2, d.lgHb) &&
                               (rowIn || !hits(T.sh, T.nb, rowLo, LGM)) && !ctrlVar;
             if (!ANYTAB || fast) {
-                const C* tb = (const C*)pre.arena + T.off + (uint32_t)((inBase >> T.sh) & ((1ull << T.nb) - 1ull));
+                // choice bits are tile-uniform here: they only move the tile's base
+                const C* tb = (const C*)pre.arena + T.off + (uint32_t)((inBase >> T.sh) & ((1ull << T.nb) - 1ull)) +
+                              (T.nsel > 0 ? tab_choice(T, inBase) : 0u);
                 const int s = rowIn ? rowLo - T.sh : 31;  // 31: the row does not index the table
                 const bool cj = T.ctrl >= 0 && ((inBase >> T.ctrl) & 1ull);
 #pragma unroll
@@ -619,12 +636,18 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                     for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
                 }
             } else if constexpr (ANYTAB) {
+                // per element; the choice bits are tile-uniform (the tables were
+                // compiled for this tiling), so the choice is one per tile
+                const C* tb = (const C*)pre.arena + T.off + (T.nsel > 0 ? tab_choice(T, inBase) : 0u);
 #pragma unroll
                 for (int j0 = 0; j0 < E; j0 += CH) {
                     C e[CH];
 #pragma unroll
-                    for (int j = 0; j < CH; ++j)
-                        e[j] = tab_entry<C>(pre.arena, T, elem((j0 + j) / RP, (j0 + j) % RP));
+                    for (int j = 0; j < CH; ++j) {
+                        const uint64_t g = elem((j0 + j) / RP, (j0 + j) % RP);
+                        e[j] = tb[(uint32_t)((g >> T.sh) & ((1ull << T.nb) - 1ull))];
+                        if (T.ctrl >= 0 && ((g >> T.ctrl) & 1ull)) e[j].y = -e[j].y;
+                    }
 #pragma unroll
                     for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
                 }
@@ -931,6 +954,12 @@ struct PassGeom {
 };
 
 static int ilog2(uint64_t v) { return 63 - __builtin_clzll(v); }
+
+// Index bits that vary within one tile of a pass: columns, rows, row blocks.
+static uint64_t tile_bits(const FftPass& d) {
+    auto field = [](int s, int n) { return n > 0 ? (((1ull << n) - 1ull) << s) : 0ull; };
+    return field(0, d.lgC) | field(d.lgInR, d.lgM) | field(d.lgInH2, d.lgHb);
+}
 static size_t lds_bytes(uint64_t tile, size_t esz) { return (tile + tile / 32) * esz; }
 
 // Tile limits: a strided tile stays within 64 KiB of LDS so several workgroups
@@ -1012,6 +1041,8 @@ static bool table_by_row(const FastDiag& pre, int fs, int fn) {
         const bool out = T.sh >= fs + fn || fs >= T.sh + T.nb;
         if (!in && !out) return false;
         if (T.ctrl >= fs && T.ctrl < fs + fn) return false;
+        for (int j = 0; j < T.nsel; ++j)
+            if (tab_sel(T, j) >= fs && tab_sel(T, j) < fs + fn) return false;
     }
     return true;
 }
@@ -1110,14 +1141,20 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     if (H == 0 || (H & (H - 1)) != 0 || H * N * S != total)
         throw CvdvError{CVDV_ERR_ARG, "FFT over a buffer whose size is not a power-of-two multiple of the field"};
     bool act = false, folded = false;
-    const FastDiag pre = compile_diag(ctx, preProg, act, outScale, &folded);
-    const int usePre = act ? 1 : 0;
     const int inv = inverse ? 1 : 0;
+    // The pending phases are compiled once the tiling of the pass that applies
+    // them is known: their tables take no choice bit that varies within a tile.
+    auto compile_pre = [&](uint64_t tileBits) {
+        const FastDiag f = compile_diag(ctx, preProg, act, outScale, &folded, tileBits);
+        return f;
+    };
 
     // single pass: rows j (stride S), lo in [0,S), hi blocks in h2 (stride N*S)
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
     FftPass d{};
     if (fit_pass(g1, prec, d, true)) {
+        const FastDiag pre = compile_pre(tile_bits(d));
+        const int usePre = act ? 1 : 0;
         d.tw4 = 0; d.tw4Shift = 0; d.tw4HiBits = 0; d.tw4LogN = 0;
         d.usePre = usePre;
         d.useScale = folded ? 0 : 1;
@@ -1129,6 +1166,9 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     }
     if (total < 4096 || N <= 64) {
         // tiny state: direct DFT, out of place, FP64 accumulation rounded once
+        // (per-element table reads: any choice bits)
+        const FastDiag pre = compile_pre(0);
+        const int usePre = act ? 1 : 0;
         const double sc = folded ? 1.0 : outScale;
         const double2* twN = (const double2*)full_table(ctx, nb);
         C* tmp = tmpFn();
@@ -1155,6 +1195,8 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     FftPass d2{};
     if (!fit_pass(p1, prec, d1, false) || !fit_pass(p2, prec, d2, false))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
+    const FastDiag pre = compile_pre(tile_bits(d1));
+    const int usePre = act ? 1 : 0;
     // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its
     // output; the twiddle T = W_N^{j1 k2} is applied by pass 2 on load, where the
     // pass has memory time to spare (pass 1 also applies the pending phases)

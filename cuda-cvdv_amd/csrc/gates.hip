@@ -7,6 +7,7 @@
 // the state (an FFT pass, a qubit-pair pass, a flush).  Parity and register
 // swap are recorded in the lazy layout map instead of moving data.
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <string>
@@ -55,36 +56,53 @@ __global__ void k_diag_prog(C* __restrict__ s, uint64_t total, DiagProg p) {
     }
 }
 
-struct TabMake {
-    int kind, nb;
-    uint32_t off;
-    int pad;
+// One term of a table: a polynomial over the table's field (vsel < 0), or a
+// one-bit term whose bit is choice bit vsel (Rz on a qubit: c1 * (+-1); a
+// polynomial over a one-qubit register: m = +-1); a control on choice bit csel
+// negates its angle.
+struct TabMember {
+    int kind, vsel, csel, pad;
     double c1, c2, c3;
-    double g;      // constant phase folded into this table (revolutions)
-    double scale;  // constant magnitude folded into this table
+};
+
+struct TabMake {
+    int nb;          // field width; entries = 2^(nb + nsel)
+    int m0, nm;      // members [m0, m0 + nm)
+    uint32_t off;
+    double g;        // constant phase folded into this table (revolutions)
+    double scale;    // constant magnitude folded into this table
 };
 
 struct TabMakeArgs {
     int n;
     uint32_t total;
-    TabMake t[CVDV_MAX_TERMS];
+    TabMake t[CVDV_FAST_TABS];
+    TabMember m[CVDV_MAX_TERMS];
 };
 
-// e^{2 pi i phase(j)} for every tabulated term, FP64 angle and sin/cos,
-// rounded once to the state's precision.
+// e^{2 pi i phase(entry)} for every table entry: each member's angle in FP64,
+// reduced and summed, then FP64 sin/cos, rounded once to the state's precision.
 template <typename C>
 __global__ void k_make_tables(C* __restrict__ arena, TabMakeArgs a) {
     for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < a.total; e += gridDim.x * blockDim.x) {
         int i = 0;
         while (i + 1 < a.n && e >= a.t[i + 1].off) ++i;
         const TabMake& t = a.t[i];
-        const uint64_t j = e - t.off;
-        double v;
-        if (t.kind == DT_POLY) {
-            const double m = (double)((int64_t)(2 * j) + 1 - ((int64_t)1 << t.nb));
-            v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
-        } else {
-            v = j ? t.c1 : -t.c1;
+        const uint32_t r = e - t.off;
+        const uint32_t loc = r & ((1u << t.nb) - 1u), choice = r >> t.nb;
+        const double mf = (double)((int64_t)(2 * loc) + 1 - ((int64_t)1 << t.nb));
+        double v = 0.0;
+        for (int k = t.m0; k < t.m0 + t.nm; ++k) {
+            const TabMember& M = a.m[k];
+            double mv;
+            if (M.vsel >= 0) {
+                const double m = ((choice >> M.vsel) & 1u) ? 1.0 : -1.0;
+                mv = M.kind == DT_ZBIT ? M.c1 * m : fma(fma(M.c3, m, M.c2), m, M.c1) * m;
+            } else {
+                mv = M.kind == DT_ZBIT ? M.c1 * mf : fma(fma(M.c3, mf, M.c2), mf, M.c1) * mf;
+            }
+            if (M.csel >= 0 && ((choice >> M.csel) & 1u)) mv = -mv;
+            v += cvdv_frac(mv);
         }
         const double2 u = unit_rev(cvdv_frac(cvdv_frac(v) + t.g), 0.0);
         arena[e] = cmk<C>(u.x * t.scale, u.y * t.scale);
@@ -230,39 +248,172 @@ void arena_reserve(CVDVContext* ctx, size_t need) {
     ctx->arenaBytes = sz;
 }
 
-FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale, bool* scaleFolded) {
-    FastDiag f{};
-    if (scaleFolded) *scaleFolded = false;
-    active = false;
-    TabMakeArgs a{};
-    uint32_t off = 0;
-    int ngen = 0;
+static bool term_tabulated(const DiagTerm& t) {
+    return (t.kind == DT_POLY && t.nb1 <= 20) || t.kind == DT_ZBIT;
+}
+
+// A program with more tabulated terms than one kernel holds (a conditional
+// term per control qubit, for instance): diagonal terms commute, so the
+// tabulated terms beyond the first CVDV_FAST_TABS - 1 go through table passes
+// of their own, CVDV_FAST_TABS at a time, and the rest (with the slot left for
+// the constant phase and scale) stays fused in the caller's kernel.  A table
+// pass reads one table entry per amplitude; the FP64 pass it replaces
+// evaluates every term's angle and sin/cos per amplitude.
+static FastDiag compile_spilled(CVDVContext* ctx, const DiagProg& p, bool& active, double scale,
+                                bool* scaleFolded, uint64_t tileBits) {
+    DiagProg head{}, pass{};
+    head.grev = p.grev;
+    int nh = 0;
     for (int i = 0; i < p.n; ++i) {
         const DiagTerm& t = p.t[i];
         if (term_is_identity(t)) continue;
-        const bool tab = (t.kind == DT_POLY && t.nb1 <= 20) || t.kind == DT_ZBIT;
-        if (tab) {
-            if (f.ntab == CVDV_FAST_TABS) { ngen = -1; break; }
-            const int nb = t.kind == DT_ZBIT ? 1 : t.nb1;
-            f.tab[f.ntab++] = DiagTab{t.sh1, nb, t.ctrl, off};
-            a.t[a.n++] = TabMake{t.kind, nb, off, 0, t.c1, t.c2, t.c3, 0.0, 1.0};
-            off += 1u << nb;
-        } else {
-            if (ngen == CVDV_FAST_GEN) { ngen = -1; break; }
-            f.gen[ngen++] = t;
+        if (!term_tabulated(t) || nh < CVDV_FAST_TABS - 1) {
+            nh += term_tabulated(t) ? 1 : 0;
+            head.t[head.n++] = t;
+            continue;
+        }
+        pass.t[pass.n++] = t;
+        if (pass.n == CVDV_FAST_TABS) {
+            launch_diag(ctx, pass);
+            pass = DiagProg{};
         }
     }
-    if (ngen < 0) {
-        // beyond the compiled capacities: one FP64 pass now, nothing left to apply
+    if (pass.n) launch_diag(ctx, pass);
+    return compile_diag(ctx, head, active, scale, scaleFolded, tileBits);
+}
+
+// Tabulated terms grouped into tables (at most CVDV_FAST_TABS + 1 groups are
+// formed; more means the program does not fit).
+struct TabGroup {
+    int sh, nb, ctrl, nsel;
+    int solo;  // a term whose choice bit would vary within a consumer's tile
+    int sel[CVDV_TAB_SEL];
+    int nm;
+    int mem[CVDV_MAX_TERMS];
+};
+
+static int sel_index(const TabGroup& g, int bit) {
+    for (int j = 0; j < g.nsel; ++j)
+        if (g.sel[j] == bit) return j;
+    return -1;
+}
+
+// Terms over the same field of two or more bits share a table when their
+// control bits lie outside the field; one-bit terms (and their controls) share
+// a field-less table over the combinations of their bits.  A table keeps at
+// most CVDV_TAB_SEL choice bits and 2^CVDV_TAB_MERGE_BITS entries, and no
+// choice bit among tileBits (such a term keeps a table of its own).
+static int group_tables(const DiagProg& p, TabGroup* g, uint64_t tileBits) {
+    int ng = 0;
+    auto add_sel = [](TabGroup& G, int bit) {
+        if (bit >= 0 && sel_index(G, bit) < 0) G.sel[G.nsel++] = bit;
+    };
+    for (int i = 0; i < p.n; ++i) {
+        const DiagTerm& t = p.t[i];
+        if (term_is_identity(t) || !term_tabulated(t)) continue;
+        const bool oneBit = t.kind == DT_ZBIT || t.nb1 == 1;
+        const bool solo = (oneBit && ((tileBits >> t.sh1) & 1ull)) || (t.ctrl >= 0 && ((tileBits >> t.ctrl) & 1ull));
+        int k = solo ? ng : 0;
+        for (; k < ng; ++k) {
+            TabGroup& G = g[k];
+            if (G.solo) continue;
+            int need = 0;
+            if (oneBit) {
+                if (G.nb != 0) continue;
+                need = (sel_index(G, t.sh1) < 0) + (t.ctrl >= 0 && t.ctrl != t.sh1 && sel_index(G, t.ctrl) < 0);
+            } else {
+                if (G.sh != t.sh1 || G.nb != t.nb1) continue;
+                if (t.ctrl >= t.sh1 && t.ctrl < t.sh1 + t.nb1) continue;
+                need = t.ctrl >= 0 && sel_index(G, t.ctrl) < 0;
+            }
+            if (G.nsel + need <= CVDV_TAB_SEL && G.nb + G.nsel + need <= CVDV_TAB_MERGE_BITS) break;
+        }
+        if (k == ng) {
+            if (ng == CVDV_FAST_TABS + 1) return ng;
+            g[ng] = TabGroup{};
+            g[ng].sh = oneBit ? 0 : t.sh1;
+            g[ng].nb = oneBit ? 0 : t.nb1;
+            g[ng].ctrl = -1;
+            g[ng].solo = solo;
+            ++ng;
+        }
+        TabGroup& G = g[k];
+        if (oneBit) add_sel(G, t.sh1);
+        if (t.ctrl >= 0) add_sel(G, t.ctrl);
+        G.mem[G.nm++] = i;
+    }
+    // A lone term keeps the plain form: its field indexes the table and its
+    // control conjugates the entry.
+    for (int k = 0; k < ng; ++k) {
+        TabGroup& G = g[k];
+        if (G.nm != 1) continue;
+        const DiagTerm& t = p.t[G.mem[0]];
+        G.sh = t.sh1;
+        G.nb = t.kind == DT_ZBIT ? 1 : t.nb1;
+        G.ctrl = t.ctrl;
+        G.nsel = 0;
+    }
+    return ng;
+}
+
+size_t diag_arena_bound(int maxNb, size_t elemBytes) {
+    const int plain = std::min(maxNb, 20);
+    const int merged = std::min(maxNb + CVDV_TAB_SEL, CVDV_TAB_MERGE_BITS);
+    return ((size_t)CVDV_FAST_TABS * ((size_t)1 << std::max(plain, merged)) + 1) * elemBytes;
+}
+
+FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale, bool* scaleFolded,
+                      uint64_t tileBits) {
+    FastDiag f{};
+    if (scaleFolded) *scaleFolded = false;
+    active = false;
+    int ngen = 0;
+    for (int i = 0; i < p.n; ++i) ngen += !term_is_identity(p.t[i]) && !term_tabulated(p.t[i]);
+    if (ngen > CVDV_FAST_GEN) {
+        // too many generic terms: one FP64 pass now, nothing left to apply
         diag_prog_pass(ctx, p);
         return FastDiag{};
     }
-    f.ngen = ngen;
-    // The global phase and the scale are folded into an unconditional table
-    // (exact in FP64 before the single rounding); a rounded constant factor
-    // would scale the norm by the same |g|^2 - 1 on every pass.  Without such a
-    // table they go to the generic terms' angle (phase only) or to a one-entry
-    // table.
+    TabGroup grp[CVDV_FAST_TABS + 1];
+    const int ng = group_tables(p, grp, tileBits);
+    if (ng > CVDV_FAST_TABS) return compile_spilled(ctx, p, active, scale, scaleFolded, tileBits);
+    for (int i = 0; i < p.n; ++i)
+        if (!term_is_identity(p.t[i]) && !term_tabulated(p.t[i])) f.gen[f.ngen++] = p.t[i];
+
+    TabMakeArgs a{};
+    uint32_t off = 0;
+    int nm = 0;
+    for (int k = 0; k < ng; ++k) {
+        const TabGroup& G = grp[k];
+        DiagTab T{};
+        T.sh = G.sh;
+        T.nb = G.nb;
+        T.ctrl = G.ctrl;
+        T.off = off;
+        T.nsel = G.nsel;
+        for (int j = 0; j < G.nsel; ++j) T.sel |= (uint32_t)G.sel[j] << (8 * j);
+        f.tab[f.ntab++] = T;
+        a.t[a.n++] = TabMake{G.nb, nm, G.nm, off, 0.0, 1.0};
+        for (int j = 0; j < G.nm; ++j) {
+            const DiagTerm& t = p.t[G.mem[j]];
+            TabMember M{};
+            M.kind = t.kind;
+            M.vsel = G.nsel > 0 && (t.kind == DT_ZBIT || t.nb1 == 1) && G.nb == 0 ? sel_index(G, t.sh1) : -1;
+            M.csel = G.nsel > 0 && t.ctrl >= 0 ? sel_index(G, t.ctrl) : -1;
+            M.c1 = t.c1;
+            M.c2 = t.c2;
+            M.c3 = t.c3;
+            a.m[nm++] = M;
+        }
+        off += 1u << (G.nb + G.nsel);
+    }
+    // The global phase and the scale are folded into a table that every
+    // element reads unconjugated (exact in FP64 before the single rounding); a
+    // rounded constant factor would scale the norm by the same |g|^2 - 1 on
+    // every pass.  Without such a table the phase goes to the generic terms'
+    // angle and the scale back to the caller (which applies it unbiased), or a
+    // lone conditional term takes its control as a choice bit and carries both,
+    // or, last, a one-entry table does.
     const double grev = cvdv_frac(p.grev);
     int host = -1;
     for (int i = 0; i < a.n; ++i)
@@ -270,17 +421,37 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double 
             host = i;
             break;
         }
+    if (host < 0 && (grev != 0.0 || scale != 1.0) && (f.ngen == 0 || (scale != 1.0 && !scaleFolded))) {
+        for (int i = 0; i < a.n; ++i) {
+            DiagTab& T = f.tab[i];
+            if (a.t[i].nm != 1 || T.ctrl < 0 || ((tileBits >> T.ctrl) & 1ull) ||
+                T.nb + 1 > CVDV_TAB_MERGE_BITS)
+                continue;
+            // entries double from here on: shift the later tables
+            const uint32_t grow = 1u << T.nb;
+            for (int j = i + 1; j < a.n; ++j) {
+                f.tab[j].off += grow;
+                a.t[j].off += grow;
+            }
+            off += grow;
+            T.sel = (uint32_t)T.ctrl;
+            T.nsel = 1;
+            T.ctrl = -1;
+            a.m[a.t[i].m0].csel = 0;
+            host = i;
+            break;
+        }
+    }
     if (host < 0 && (grev != 0.0 || scale != 1.0)) {
-        if (scale == 1.0 && f.ngen > 0) {
+        if (f.ngen > 0 && (scale == 1.0 || scaleFolded)) {
             f.genrev = grev;
         } else if (f.ntab < CVDV_FAST_TABS) {
             host = f.ntab;
-            f.tab[f.ntab++] = DiagTab{0, 0, -1, off};
-            a.t[a.n++] = TabMake{DT_POLY, 0, off, 0, 0.0, 0.0, 0.0, 0.0, 1.0};
+            f.tab[f.ntab++] = DiagTab{0, 0, -1, off, 0, 0u};
+            a.t[a.n++] = TabMake{0, nm, 0, off, 0.0, 1.0};
             off += 1;
         } else {
-            diag_prog_pass(ctx, p);
-            return FastDiag{};
+            return compile_spilled(ctx, p, active, scale, scaleFolded, tileBits);
         }
     }
     if (host >= 0) {

@@ -143,6 +143,52 @@ def test_random_circuit_parity(regs, seed, prec):
     assert abs(h.m() - o.m()) <= (1e-11 if prec == "complex128" else 2e-5)
 
 
+def _spill_case(kind):
+    """Pending phase programs with many tabulated terms: more conditional tables
+    than one kernel holds, or terms that share tables.
+
+    A conditional displacement with an imaginary amplitude is a phase e^{i k x}
+    on the target, conditioned on the control qubit: one tabulated term per
+    (target, control) pair.  'fft': eight such terms on one oscillator, consumed
+    by its FFT.  'qubit': seven terms controlled by one qubit (after an FFT left
+    a global phase pending), consumed by a Hadamard on that qubit.  'host': six
+    conditional terms and the global phase (no unconditional table to carry it).
+    'bits': Rz on six qubits of two registers (one-bit terms on different bits,
+    read through shared tables) and kicks on the oscillator controlled by each
+    of them, consumed by the oscillator's FFT and by the final flush.
+    """
+    if kind == "bits":
+        regs = [3, 3, 7]
+        init = [["setUniform", 0], ["setUniform", 1], ["setCoherent", 2, [0.6, 0.3]]]
+        ops = [["rz", r, q, 0.3 + 0.4 * q + r] for r in (0, 1) for q in range(3)]
+        ops += [["cd", 2, r, q, [0.0, 0.1 + 0.05 * q + 0.2 * r]] for r in (0, 1) for q in range(3)]
+        ops += [["ftQ2P", 2], ["rz", 0, 1, 0.7]]
+        return regs, init, ops
+    if kind == "fft":
+        regs = [1] * 8 + [6]
+        init = [["setUniform", q] for q in range(8)] + [["setCoherent", 8, [0.5, -0.2]]]
+        ops = [["cd", 8, q, 0, [0.0, 0.15 * (q + 1)]] for q in range(8)] + [["ftQ2P", 8]]
+    else:
+        n = 7 if kind == "qubit" else 6
+        regs = [1] + [3] * n
+        init = [["setUniform", 0]] + [["setCoherent", r, [0.3, 0.1 * r]] for r in range(1, n + 1)]
+        ops = [["ftQ2P", 1]] + [["cd", r, 0, 0, [0.0, 0.2 * r]] for r in range(1, n + 1)] + [["h", 0, 0]]
+    return regs, init, ops
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("kind", ["fft", "qubit", "host", "bits"])
+def test_spilled_phase_tables(kind, prec):
+    print(f"[gpu] spilled phase tables {kind} {prec}", flush=True)
+    regs, init, ops = _spill_case(kind)
+    h, o = run_pair(regs, init, ops, prec)
+    hs, os_ = h.getState(), o.getState()
+    err = np.max(np.abs(hs - os_)) / np.max(np.abs(os_))
+    print(f"    rel max amp err {err:.3e}")
+    assert err <= amp_tol(prec)
+    assert fidelity_loss(hs, os_) <= (1e-13 if prec == "complex128" else 1e-6)
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_readouts_after_lazy_layout(prec):
     """Wigner/Husimi/marginals/fidelity while registers are swapped and reflected."""

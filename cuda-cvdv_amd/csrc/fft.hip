@@ -531,7 +531,8 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
 // has checked that every table is read by row, table_by_row()).
 template <int RP, int E, int LGM, int LGC, bool TW4, bool ANYTAB, typename C>
 __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
-                                           const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt) {
+                                           const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt,
+                                           C* sRow) {
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
@@ -594,17 +595,25 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         // 64-bit flat index.
         const int rowLo = d.lgInR, rowHi = d.lgInR + LGM;
         auto hits = [](int sh, int nb, int lo, int n) { return n > 0 && sh < lo + n && lo < sh + nb; };
-        auto apply_table = [&](const DiagTab& TL) {
+        // Where a table is read through the tile's base: the base, the row shift
+        // (31: the row does not index it) and the conjugation.
+        struct TabAt {
+            DiagTab T;
+            bool fast, cj;
+            int s;
+            const C* tb;
+        };
+        auto tab_at = [&](const DiagTab& TL) {
+            TabAt A;
             // the descriptor is read from LDS into vector registers; it is the
             // same for every lane, so keep it in scalar registers
-            DiagTab T;
+            DiagTab& T = A.T;
             T.sh = __builtin_amdgcn_readfirstlane(TL.sh);
             T.nb = __builtin_amdgcn_readfirstlane(TL.nb);
             T.ctrl = __builtin_amdgcn_readfirstlane(TL.ctrl);
             T.off = (uint32_t)__builtin_amdgcn_readfirstlane((int)TL.off);
             T.nsel = __builtin_amdgcn_readfirstlane(TL.nsel);
             T.sel = (uint32_t)__builtin_amdgcn_readfirstlane((int)TL.sel);
-            constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
             const bool rowIn = T.sh <= rowLo && rowHi <= T.sh + T.nb;
             auto varies = [&](int b) {
                 return b >= 0 && (b < LGC || (b >= rowLo && b < rowHi) ||
@@ -614,14 +623,24 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
 #pragma unroll
             for (int j = 0; j < CVDV_TAB_SEL; ++j)
                 if (j < T.nsel) ctrlVar = ctrlVar || varies(tab_sel(T, j));
-            const bool fast = !hits(T.sh, T.nb, 0, LGC) && !hits(T.sh, T.nb, d.lgInH2, d.lgHb) &&
-                              (rowIn || !hits(T.sh, T.nb, rowLo, LGM)) && !ctrlVar;
+            A.fast = !hits(T.sh, T.nb, 0, LGC) && !hits(T.sh, T.nb, d.lgInH2, d.lgHb) &&
+                     (rowIn || !hits(T.sh, T.nb, rowLo, LGM)) && !ctrlVar;
+            // choice bits are tile-uniform here: they only move the tile's base
+            A.tb = (const C*)pre.arena + T.off + (uint32_t)((inBase >> T.sh) & ((1ull << T.nb) - 1ull)) +
+                   (T.nsel > 0 ? tab_choice(T, inBase) : 0u);
+            A.s = rowIn ? rowLo - T.sh : 31;
+            A.cj = T.ctrl >= 0 && ((inBase >> T.ctrl) & 1ull);
+            return A;
+        };
+        auto apply_table = [&](const DiagTab& TL) {
+            const TabAt A = tab_at(TL);
+            const DiagTab& T = A.T;
+            const bool fast = A.fast;
+            constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
             if (!ANYTAB || fast) {
-                // choice bits are tile-uniform here: they only move the tile's base
-                const C* tb = (const C*)pre.arena + T.off + (uint32_t)((inBase >> T.sh) & ((1ull << T.nb) - 1ull)) +
-                              (T.nsel > 0 ? tab_choice(T, inBase) : 0u);
-                const int s = rowIn ? rowLo - T.sh : 31;  // 31: the row does not index the table
-                const bool cj = T.ctrl >= 0 && ((inBase >> T.ctrl) & 1ull);
+                const C* tb = A.tb;
+                const int s = A.s;
+                const bool cj = A.cj;
 #pragma unroll
                 for (int j0 = 0; j0 < E; j0 += CH) {
                     C e[CH];
@@ -653,11 +672,44 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 }
             }
         };
-        if (pre.ntab > 0) apply_table(P.tab[0]);
-        for (int t = 1; t < pre.ntab; ++t) {
-            // opaque: the element indices are recomputed per table, not held
-            asm volatile("" : "+v"(tt));
-            apply_table(P.tab[t]);
+        bool staged = false;
+        if constexpr (LGC > 0 && ANYTAB) {
+            // Tiles of several columns: when every table is read by row, their
+            // product over the tile's M rows is staged in LDS (one table load per
+            // row instead of one per element), and each element takes one LDS
+            // read and one product.  The exchange buffer is free before the
+            // stages; a barrier hands it back.
+            bool byRow = pre.ntab > 0;
+            for (int t = 0; t < pre.ntab; ++t) byRow = byRow && tab_at(P.tab[t]).fast;
+            if (byRow) {
+                for (int r = tid; r < (1 << LGM); r += nt) {
+                    C f = cmk<C>(1.0, 0.0);
+                    for (int t = 0; t < pre.ntab; ++t) {
+                        const TabAt A = tab_at(P.tab[t]);
+                        C e = A.tb[A.s < 31 ? ((uint32_t)r << A.s) : 0u];
+                        if (A.cj) e.y = -e.y;
+                        f = t == 0 ? e : cmul(f, e);
+                    }
+                    sRow[r] = f;
+                }
+                lds_barrier();
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+#pragma unroll
+                    for (int i = 0; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], sRow[p.q + (i << lgMq)]);
+                }
+                lds_barrier();
+                staged = true;
+            }
+        }
+        if (!staged) {
+            if (pre.ntab > 0) apply_table(P.tab[0]);
+            for (int t = 1; t < pre.ntab; ++t) {
+                // opaque: the element indices are recomputed per table, not held
+                asm volatile("" : "+v"(tt));
+                apply_table(P.tab[t]);
+            }
         }
         if (pre.ngen > 0) {
             asm volatile("" : "+v"(tt));
@@ -816,7 +868,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     auto prephase = [&](C* x, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        stage0_pre<R0, E, LGM, LGC, !PF, !PF>(x, d, inBase, P, pre, tw4, ti, nt);
+        stage0_pre<R0, E, LGM, LGC, !PF, !PF>(x, d, inBase, P, pre, tw4, ti, nt, lds);
     };
     // Transform the phased tile in registers (the last stage's outputs stay in x).
     auto stages = [&](C* x, int ti, const C* twt) {

@@ -129,6 +129,32 @@ __global__ void k_merge_splits(const double* __restrict__ part, double* __restri
     }
 }
 
+// Many splits and few bins (a qubit's marginal: 2 bins, thousands of splits):
+// one workgroup per bin, each thread summing a strided share of the partials,
+// then a fixed-order tree over the threads (compensated; deterministic).
+__global__ void __launch_bounds__(256) k_merge_splits_bin(const double* __restrict__ part,
+                                                          double* __restrict__ out, uint64_t outSize,
+                                                          int usplit) {
+    __shared__ double ss[256];
+    __shared__ double cc[256];
+    const uint64_t o = blockIdx.x;
+    const int tid = threadIdx.x;
+    double s = 0.0, c = 0.0;
+    for (int k = tid; k < usplit; k += 256) ksum_add(s, c, part[(uint64_t)k * outSize + o]);
+    ss[tid] = s;
+    cc[tid] = c;
+    for (int w = 128; w > 0; w >>= 1) {
+        __syncthreads();
+        if (tid < w) {
+            double s2 = ss[tid], c2 = cc[tid] + cc[tid + w];
+            ksum_add(s2, c2, ss[tid + w]);
+            ss[tid] = s2;
+            cc[tid] = c2;
+        }
+    }
+    if (tid == 0) out[o] = ss[0] + cc[0];
+}
+
 static int popc(uint64_t m) { return __builtin_popcountll(m); }
 
 // Marginal over `regs` (logical registers; output column-major, first fastest).
@@ -178,10 +204,15 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
     prof_end(ctx, pid);
     const double* res = part;
     if (usplit > 1) {
-        uint64_t b = (outSize + 255) / 256;
-        if (b > 4096) b = 4096;
-        hipLaunchKernelGGL(k_merge_splits, dim3((unsigned)b), dim3(256), 0, ctx->stream, part, fin,
-                           outSize, (int)usplit);
+        if (usplit >= 256 && outSize <= 4096) {
+            hipLaunchKernelGGL(k_merge_splits_bin, dim3((unsigned)outSize), dim3(256), 0, ctx->stream, part,
+                               fin, outSize, (int)usplit);
+        } else {
+            uint64_t b = (outSize + 255) / 256;
+            if (b > 4096) b = 4096;
+            hipLaunchKernelGGL(k_merge_splits, dim3((unsigned)b), dim3(256), 0, ctx->stream, part, fin,
+                               outSize, (int)usplit);
+        }
         HIPCHK(hipGetLastError());
         res = fin;
     }

@@ -118,7 +118,10 @@ def _random_ops(rng, regs, n):
                                        ([1, 2, 2], 20),
                                        # two-pass FFTs of a 2^16 contiguous register (narrowed
                                        # pass-1 tiles) and of a strided 2^13 register
-                                       ([1, 16], 21), ([1, 13, 2], 22)])
+                                       ([1, 16], 21), ([1, 13, 2], 22),
+                                       # a contiguous 2^14 register: one-row tiles in complex64,
+                                       # the two-pass split in complex128 (256 KiB rows)
+                                       ([1, 14], 23), ([3, 14], 24)])
 def test_random_circuit_parity(regs, seed, prec):
     print(f"[gpu] random {regs} seed {seed} {prec}", flush=True)
     rng = np.random.default_rng(seed)

@@ -639,6 +639,7 @@ def main():
     ap.add_argument("--c4-only", action="store_true", help="run only the batched-circuit workload")
     ap.add_argument("--no-c3", action="store_true", help="skip the state-transfer workload")
     ap.add_argument("--c3-only", action="store_true", help="run only the state-transfer workload")
+    ap.add_argument("--c15-only", action="store_true", help="run only the phase-space readout workloads (C1, C5)")
     ap.add_argument("--no-c5", action="store_true", help="skip the phase-space readout workloads (C1, C5)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the plain-copy bandwidth probe (counter calibration runs)")
@@ -661,6 +662,9 @@ def main():
         return
     if args.c3_only:
         print(json.dumps(c3_section(max(1, args.steps))))
+        return
+    if args.c15_only:
+        print(json.dumps({"c1": c1_section(max(1, args.steps)), "c5": c5_section(max(1, args.steps))}))
         return
     ops = make_circuit()
     passes = sum(ft_count(op) for op in ops)

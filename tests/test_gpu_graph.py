@@ -77,3 +77,38 @@ def test_graph_capture_refuses_readouts_and_recovers():
         a.h(0, 0)
     g.launch()
     np.testing.assert_allclose(a.getState(), before, atol=1e-14)
+
+
+def _many_controls(sim, nq):
+    """More controlled phases on the oscillator than one kernel's tables hold (a
+    spilled table pass), Rz on every qubit (shared one-bit tables), FFTs whose
+    tables take the control bits as choice bits, and a qubit gate in between."""
+    osc = nq
+    for q in range(nq):
+        sim.cd(osc, q, 0, 0.1j * (q + 1))
+        sim.rz(q, 0, 0.3 + 0.2 * q)
+    sim.ftQ2P(osc)
+    sim.h(0, 0)
+    for q in range(nq):
+        sim.cd(osc, q, 0, 0.05 * (q + 1))
+    sim.ftP2Q(osc)
+
+
+@pytest.mark.parametrize("prec", ["complex128", "mixed"])
+def test_graph_replays_shared_and_spilled_tables(prec):
+    print(f"[graph] shared and spilled phase tables {prec}", flush=True)
+    nq = 8
+    regs = [1] * nq + [6]
+    a = _sim(regs, prec)
+    b = _sim(regs, prec)
+    with a.capture() as g:
+        _many_controls(a, nq)
+    np.testing.assert_array_equal(a.getState(), b.getState())
+    for rep in range(3):
+        g.launch()
+        _many_controls(b, nq)
+        sa, sb = a.getState(), b.getState()
+        scale = np.max(np.abs(sb))
+        assert np.max(np.abs(sa - sb)) <= (1e-13 if prec == "complex128" else 1e-6) * scale, rep
+        a.rz(1, 0, 0.4)  # eager work between replays
+        b.rz(1, 0, 0.4)

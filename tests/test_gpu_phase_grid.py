@@ -73,7 +73,7 @@ def _branches(prec, seed=5, nq=(2, 2), cv=9, ngates=20):
 
 @pytest.mark.parametrize("prec", ["complex128", "mixed"])
 def test_c1_wigner_201_grid(prec):
-    print(f"[grid] C1 201x201 Wigner {prec}", flush=True)
+    print(f"[grid] [1,10] Wigner on 201x201 {prec}", flush=True)
     h, o = _c1(prec)
     xs = ps = np.linspace(-5, 5, 201)
     tol = 1e-12 if prec == "complex128" else 2e-5
@@ -123,9 +123,9 @@ def test_per_branch_against_oracle_and_linearity(prec):
 
 
 def test_c5_512_grid_subset_against_oracle():
-    """C5 shape: [2,2,12], 16 branches, 512 x 512 over [-6,6]^2; the full grid on the
+    """[2,2,12], 16 branches, 512 x 512 over [-6,6]^2; the full grid on the
     GPU, checked against the oracle on 12 columns (every branch, every row)."""
-    print("[grid] C5 512x512 per branch", flush=True)
+    print("[grid] [2,2,12] 512x512 per branch", flush=True)
     h, o = _branches("complex128", seed=7, cv=12)
     xs = ps = np.linspace(-6, 6, 512)
     Wb = h.getWignerGrid(2, xs, ps, per_slice=True)

@@ -1,5 +1,5 @@
-"""The measured workloads at oracle-sized shapes: the state-transfer circuit (C3
-generator of bench.py at [1]*6 + [10]) through the C ABI against the oracle:
+"""The measured workloads at oracle-sized shapes: the state-transfer circuit
+(bench.py's generator at [1]*6 + [10]) through the C ABI against the oracle:
 final state, the six single-qubit marginals and the joint six-qubit marginal."""
 
 import numpy as np

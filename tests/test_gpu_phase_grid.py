@@ -30,7 +30,7 @@ def _pair(regs, prec, build):
     return h, o
 
 
-def _c1(prec):
+def _qubit_oscillator_1_10(prec):
     """[1,10]: qubit |0>, coherent 2+1i; h, cd(1.5), r(0.7), s(0.3), h."""
     def build(sep, sepn):
         for s in (sep, sepn):
@@ -72,9 +72,9 @@ def _branches(prec, seed=5, nq=(2, 2), cv=9, ngates=20):
 
 
 @pytest.mark.parametrize("prec", ["complex128", "mixed"])
-def test_c1_wigner_201_grid(prec):
+def test_wigner_201_grid_qubit_oscillator(prec):
     print(f"[grid] [1,10] Wigner on 201x201 {prec}", flush=True)
-    h, o = _c1(prec)
+    h, o = _qubit_oscillator_1_10(prec)
     xs = ps = np.linspace(-5, 5, 201)
     tol = 1e-12 if prec == "complex128" else 2e-5
     for native in (True, False):
@@ -122,7 +122,7 @@ def test_per_branch_against_oracle_and_linearity(prec):
     assert np.all(Qb >= 0)
 
 
-def test_c5_512_grid_subset_against_oracle():
+def test_512_grid_16_branches_subset_against_oracle():
     """[2,2,12], 16 branches, 512 x 512 over [-6,6]^2; the full grid on the
     GPU, checked against the oracle on 12 columns (every branch, every row)."""
     print("[grid] [2,2,12] 512x512 per branch", flush=True)

@@ -528,8 +528,13 @@ def cpu_baseline(ops, budget_s):
 def copy_probe(reps=10):
     """Plain device copy of one complex64 state (2^28 amplitudes, 2 GiB read + 2 GiB
     written per copy): the achievable-bandwidth reference for the FFT passes, and
-    the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters."""
+    the known-size kernel that calibrates the FETCH_SIZE / WRITE_SIZE counters.
+    Also the library's pattern-shaped copies (cvdvCopyProbe: each FFT pass's
+    access pattern with no arithmetic), the per-pass ceilings."""
+    import ctypes
+
     import torch
+    from hipcvdv import _lib
 
     A = 1 << sum(REGS)
     src = torch.zeros(A, dtype=torch.complex64, device="cuda")
@@ -545,7 +550,17 @@ def copy_probe(reps=10):
     ms = e0.elapsed_time(e1) / reps
     del src, dst
     torch.cuda.empty_cache()
-    return {"bytes_per_copy": 2 * A * 8, "ms": ms, "gbps": 2 * A * 8 / (ms * 1e-3) / 1e9}
+    nbytes = 2 * A * 8
+    patterns = {}
+    lib = _lib.get_lib()
+    for name, pat in _lib.PROBE_PATTERNS.items():
+        v = ctypes.c_double(0.0)
+        if lib.cvdvCopyProbe(pat, reps, ctypes.byref(v)) != 0:
+            _lib.check()
+        patterns[name] = {"ms": v.value, "gbps": nbytes / (v.value * 1e-3) / 1e9}
+        print(f"[bench] copy probe {name}: {v.value:.3f} ms, {patterns[name]['gbps']:.0f} GB/s",
+              file=sys.stderr, flush=True)
+    return {"bytes_per_copy": nbytes, "ms": ms, "gbps": nbytes / (ms * 1e-3) / 1e9, "patterns": patterns}
 
 
 def dft_gemm_probe(reps=5):

@@ -414,6 +414,7 @@ struct CVDVBatch {
     int compiledThreads = 0, compiledLds = 0;
     std::vector<size_t> seenOps;
     std::vector<DiagProg> seenPending;
+    std::vector<std::vector<int>> seenLayout;  // pshift and refl of every register
     size_t totalOps = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -448,12 +449,24 @@ static int ilog2(uint64_t v) {
     return r;
 }
 
+// The lazy layout of a circuit: parity and register swaps change only this,
+// and the compiled circuit carries it (the canonical store undoes it).
+static std::vector<int> layout_of(const CVDVContext* c) {
+    std::vector<int> v(2 * c->numReg);
+    for (int r = 0; r < c->numReg; ++r) {
+        v[2 * r] = c->pshift[r];
+        v[2 * r + 1] = c->refl[r];
+    }
+    return v;
+}
+
 static bool recording_changed(CVDVBatch* b) {
     if (!b->compiled || b->compiledThreads != b->threads || b->compiledLds != b->ldsBytes) return true;
     for (int i = 0; i < b->n; ++i) {
         const CVDVContext* c = b->circ[i];
         if (c->rec->ops.size() != b->seenOps[i]) return true;
         if (std::memcmp(&c->pending, &b->seenPending[i], sizeof(DiagProg)) != 0) return true;
+        if (layout_of(c) != b->seenLayout[i]) return true;
     }
     return false;
 }
@@ -599,9 +612,11 @@ static void batch_compile(CVDVBatch* b) {
 
     b->seenOps.resize(b->n);
     b->seenPending.resize(b->n);
+    b->seenLayout.resize(b->n);
     for (int i = 0; i < b->n; ++i) {
         b->seenOps[i] = b->circ[i]->rec->ops.size();
         b->seenPending[i] = b->circ[i]->pending;
+        b->seenLayout[i] = layout_of(b->circ[i]);
     }
     b->compiled = true;
     b->compiledThreads = b->threads;

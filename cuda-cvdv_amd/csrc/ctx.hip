@@ -181,8 +181,9 @@ __global__ void __launch_bounds__(256) k_init_product(C* __restrict__ out, uint6
     for (uint64_t c = blockIdx.x; c < nchunks; c += gridDim.x) {
         const uint64_t g0 = c << k;
         double2 hp = make_double2(1.0, 0.0);
+        // a 0-qubit register at bit k is in the LDS table already (sh + nb <= k)
         for (int r = 0; r < a.nreg; ++r)
-            if (a.sh[r] >= k) hp = cmul(hp, a.arr[r][(g0 >> a.sh[r]) & ((1ull << a.nb[r]) - 1)]);
+            if (a.sh[r] >= k && a.nb[r] > 0) hp = cmul(hp, a.arr[r][(g0 >> a.sh[r]) & ((1ull << a.nb[r]) - 1)]);
 #pragma unroll 4
         for (int l = threadIdx.x; l < L; l += blockDim.x) {
             double2 p = cmul(hp, lo[l]);
@@ -307,12 +308,6 @@ struct CVDVGraph {
     int generation;
 };
 
-struct CaptureSave {
-    int pshift[CVDV_MAX_REG], refl[CVDV_MAX_REG];
-    DiagProg pending;
-    int cur;
-};
-static thread_local CaptureSave g_capSave;
 
 // Copy the state between the two buffers (a kernel, so it can sit in a graph).
 static void copy_to(CVDVContext* ctx, int dst) {
@@ -609,11 +604,11 @@ void cvdvGraphBegin(CVDVContext* ctx) {
     fft_prepare(ctx);
     HIPCHK(hipStreamSynchronize(ctx->stream));
     for (int r = 0; r < ctx->numReg; ++r) {
-        g_capSave.pshift[r] = ctx->pshift[r];
-        g_capSave.refl[r] = ctx->refl[r];
+        ctx->capSave.pshift[r] = ctx->pshift[r];
+        ctx->capSave.refl[r] = ctx->refl[r];
     }
-    g_capSave.pending = ctx->pending;
-    g_capSave.cur = ctx->cur;
+    ctx->capSave.pending = ctx->pending;
+    ctx->capSave.cur = ctx->cur;
     HIPCHK(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
     ctx->capturing = true;
     CVDV_CATCH()
@@ -630,7 +625,7 @@ CVDVGraph* cvdvGraphEnd(CVDVContext* ctx) {
     std::string why;
     try {
         ctx_materialize(ctx);
-        if (ctx->cur != g_capSave.cur) copy_to(ctx, g_capSave.cur);
+        if (ctx->cur != ctx->capSave.cur) copy_to(ctx, ctx->capSave.cur);
     } catch (const CvdvError& e) {
         ok = false;
         why = e.msg;
@@ -639,11 +634,11 @@ CVDVGraph* cvdvGraphEnd(CVDVContext* ctx) {
     const hipError_t ce = hipStreamEndCapture(ctx->stream, &g);
     // the host-side layout returns to the capture start: nothing has executed
     for (int r = 0; r < ctx->numReg; ++r) {
-        ctx->pshift[r] = g_capSave.pshift[r];
-        ctx->refl[r] = g_capSave.refl[r];
+        ctx->pshift[r] = ctx->capSave.pshift[r];
+        ctx->refl[r] = ctx->capSave.refl[r];
     }
-    ctx->pending = g_capSave.pending;
-    ctx->cur = g_capSave.cur;
+    ctx->pending = ctx->capSave.pending;
+    ctx->cur = ctx->capSave.cur;
     if (!ok || ce != hipSuccess || !g) {
         if (g) (void)hipGraphDestroy(g);
         cvdv_set_error(CVDV_ERR_HIP, "cvdvGraphEnd: capture failed" + (why.empty() ? std::string() : ": " + why));
@@ -656,7 +651,7 @@ CVDVGraph* cvdvGraphEnd(CVDVContext* ctx) {
         return nullptr;
     }
     ctx->graphs++;
-    return new CVDVGraph{ctx, g, e, g_capSave.cur, ctx->generation};
+    return new CVDVGraph{ctx, g, e, ctx->capSave.cur, ctx->generation};
 }
 
 void cvdvGraphLaunch(CVDVGraph* gr) {

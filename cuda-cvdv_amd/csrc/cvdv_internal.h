@@ -172,6 +172,14 @@ struct ProfRec {
     hipEvent_t a, b;
 };
 
+// Host-side state at the start of a graph capture (restored at its end: nothing
+// has executed).  Kept per context, so captures on several contexts may nest.
+struct CaptureSave {
+    int pshift[CVDV_MAX_REG], refl[CVDV_MAX_REG];
+    DiagProg pending;
+    int cur;
+};
+
 struct CVDVContext {
     int numReg = 0;
     int totalQbt = 0;
@@ -195,6 +203,7 @@ struct CVDVContext {
     BatchRecording* rec = nullptr;    // non-null: record ops, launch nothing
     void* blas = nullptr;             // rocBLAS handle of the grid readouts (created on first use)
     bool capturing = false;           // between cvdvGraphBegin and cvdvGraphEnd
+    CaptureSave capSave{};            // host-side state at cvdvGraphBegin
     int graphs = 0;                   // graphs captured (they hold arena pointers)
     std::vector<void*> retired;       // arenas outgrown while a graph may still read them
     int generation = 0;               // bumped when the state buffers are released

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <map>
+#include <mutex>
 #include <tuple>
 #include <vector>
 
@@ -260,15 +261,21 @@ static const void* full_table(CVDVContext* ctx, int logN, int prec = CVDV_C128) 
     return d;
 }
 
+// Device properties the planner asks for on every launch, cached per device.
+// Contexts may run on several host threads: the caches are guarded.
+static std::mutex g_devCacheMu;
+
 size_t device_max_lds() {
-    static size_t cached = 0;
-    if (!cached) {
-        int dev = 0, v = 0;
-        HIPCHK(hipGetDevice(&dev));
-        HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-        cached = (size_t)v;
-    }
-    return cached;
+    static std::map<int, size_t> cached;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_devCacheMu);
+    auto it = cached.find(dev);
+    if (it != cached.end()) return it->second;
+    int v = 0;
+    HIPCHK(hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    cached[dev] = (size_t)v;
+    return (size_t)v;
 }
 
 // ---------------------------------------------------------------- pass descriptor
@@ -1059,15 +1066,20 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
 
 // Workgroups of a kernel instance resident on the whole device at once.
 static uint64_t resident_blocks(const void* kfn, int threads, size_t lds) {
-    static std::map<std::tuple<const void*, int, size_t>, uint64_t> cache;
-    const auto key = std::make_tuple(kfn, threads, lds);
-    auto it = cache.find(key);
-    if (it != cache.end()) return it->second;
-    int dev = 0, cus = 0, per = 0;
+    static std::map<std::tuple<int, const void*, int, size_t>, uint64_t> cache;
+    int dev = 0;
     HIPCHK(hipGetDevice(&dev));
+    const auto key = std::make_tuple(dev, kfn, threads, lds);
+    {
+        std::lock_guard<std::mutex> lk(g_devCacheMu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int cus = 0, per = 0;
     HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kfn, threads, lds));
     const uint64_t n = (uint64_t)std::max(per, 1) * (uint64_t)std::max(cus, 1);
+    std::lock_guard<std::mutex> lk(g_devCacheMu);
     cache[key] = n;
     return n;
 }

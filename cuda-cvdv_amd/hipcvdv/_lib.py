@@ -91,6 +91,17 @@ SIGNATURES = {
     "cvdvBatchMeasure": ([c_void_p, c_int, POINTER(c_double)], None),
     "cvdvBatchNorms": ([c_void_p, POINTER(c_double)], None),
     "cvdvBatchGetState": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], None),
+    "cvdvCopyProbe": ([c_int, c_int, POINTER(c_double)], c_int),
+}
+
+# cvdvCopyProbe patterns (include/cvdv_hip.h)
+PROBE_PATTERNS = {
+    "float4 copy, grid 4 x CUs x 256": 0,
+    "hipMemcpy device to device": 1,
+    "strided FFT pass (128 x 256 B rows, stride 16 MiB, in place)": 2,
+    "transposing FFT pass (rows stride 128 KiB -> stride 16 MiB)": 3,
+    "mid-digit FFT pass (128 x 256 B rows, stride 128 KiB, in place)": 4,
+    "one-row FFT pass (128 KiB row per workgroup, 1 per CU)": 5,
 }
 
 

@@ -173,6 +173,23 @@ void   cvdvBatchMeasure(CVDVBatch* batch, int regIdx, double* probsOut);
 void   cvdvBatchNorms(CVDVBatch* batch, double* normsOut);
 void   cvdvBatchGetState(CVDVBatch* batch, int i, double* realOut, double* imagOut);
 
+/* ---- copy-bandwidth probe (addition; measurement only) ----
+ * Times a plain copy of one 2^28-amplitude complex64 state (2 GiB read + 2 GiB
+ * written, buffers allocated and freed inside) with the access pattern of one
+ * FFT pass of a [14,14] state and no arithmetic; writes the mean milliseconds
+ * per copy over reps launches (after one warmup) to msOut.  Returns 0 or an
+ * error code.  Patterns: */
+enum {
+    CVDV_PROBE_COPY = 0,      /* float4 grid-stride copy (the counter calibration kernel) */
+    CVDV_PROBE_MEMCPY = 1,    /* hipMemcpyAsync device to device */
+    CVDV_PROBE_STRIDED = 2,   /* 128 rows x 256-byte runs at stride 16 MiB, in place */
+    CVDV_PROBE_TRANSPOSE = 3, /* rows at stride 128 KiB written at stride 16 MiB, out of place */
+    CVDV_PROBE_MID = 4,       /* rows at stride 128 KiB, in place */
+    CVDV_PROBE_ROW = 5,       /* one contiguous 128 KiB row per workgroup, one workgroup per CU */
+    CVDV_PROBE_COUNT = 6
+};
+int    cvdvCopyProbe(int pattern, int reps, double* msOut);
+
 #ifdef __cplusplus
 }
 #endif

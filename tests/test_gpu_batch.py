@@ -186,6 +186,29 @@ def test_batch_rerun_and_extend():
     assert np.max(np.abs(b.getState(1) - seq[0][0])) < 1e-11
 
 
+def test_batch_extend_with_layout_only_gates():
+    """Parity and swap change only the lazy layout of a recording: a rerun after
+    extending a circuit with them (right after an FFT, so the pending program is
+    unchanged) must recompile, not replay the previous launch."""
+    print("[batch] extend with parity / swap", flush=True)
+    shapes = [[1, 8], [2, 6, 6]]
+    b, circuits = _make("complex128", seed=13, n_gates=8, shapes=shapes)
+    b.run()
+    extra = {0: [("ftQ2P", 1), ("p", 1)], 1: [("ftQ2P", 1), ("swap", 1, 2), ("ftP2Q", 2), ("p", 2)]}
+    for i, ops in extra.items():
+        for g in ops:
+            getattr(b.circuit(i), g[0])(*g[1:])
+    b.run()
+    seq = _sequential("complex128", [(c[0], c[1], c[2] + extra[i]) for i, c in enumerate(circuits)])
+    for i in range(len(shapes)):
+        assert np.max(np.abs(b.getState(i) - seq[i][0])) < 1e-11, shapes[i]
+    # one more layout-only gate: the pending program is still unchanged
+    b.circuit(1).swap(1, 2)
+    b.run()
+    seq = _sequential("complex128", [(circuits[1][0], circuits[1][1], circuits[1][2] + extra[1] + [("swap", 1, 2)])])
+    assert np.max(np.abs(b.getState(1) - seq[0][0])) < 1e-11
+
+
 def test_batch_autotune_keeps_results():
     print("[batch] autotune", flush=True)
     b, circuits = _make("mixed", seed=9, n_gates=20)

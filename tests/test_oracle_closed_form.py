@@ -174,6 +174,89 @@ def test_conditional_beam_splitter_identity(nf, theta):
     assert abs(s.getFidelity(sep) - 1) < 1e-10
 
 
+# ---------------------------------------------------------------- 'cuda'-convention branches
+# The golden vectors hold TorchCvdv's branches only.  The two places where the
+# CUDA ABI the library mirrors decides differently (turn counts at exact ties,
+# and q1q2 with reg1 > reg2 of unequal sizes) are pinned here by identities
+# that hold whichever decomposition a branch takes.
+
+def test_cuda_q1q2_reversed_unequal_is_symmetric_and_brute_force():
+    """cuda q1q2 (gates.cuh:213-231) applies e^{i c q1 q2} per element, so swapping
+    the register arguments changes nothing; checked against a direct double loop."""
+    regs = [5, 7]
+    st = RNG.normal(size=2 ** 12) + 1j * RNG.normal(size=2 ** 12)
+    c = 0.37
+    outs = []
+    for r1, r2 in ((0, 1), (1, 0)):
+        s = CvdvOracle(regs, convention="cuda")
+        s.initFromArray(st)
+        s.q1q2(r1, r2, c)
+        outs.append(s.getState())
+    np.testing.assert_allclose(outs[0], outs[1], atol=1e-13)
+    # direct evaluation from the grid definition x_j = (j - (N-1)/2) sqrt(2 pi / N)
+    x = [(np.arange(1 << n) - ((1 << n) - 1) / 2) * sqrt(2 * pi / (1 << n)) for n in regs]
+    ref = st.reshape(32, 128).copy()
+    for j0 in range(32):
+        for j1 in range(128):
+            ref[j0, j1] *= np.exp(1j * c * x[0][j0] * x[1][j1])
+    np.testing.assert_allclose(outs[1], ref.reshape(-1), atol=1e-13)
+    # TorchCvdv's reshape (torch convention) differs for unequal sizes, as documented
+    t = CvdvOracle(regs, convention="torch")
+    t.initFromArray(st / np.linalg.norm(st))
+    t.q1q2(1, 0, c)
+    assert np.max(np.abs(t.getState() - ref.reshape(-1) / np.linalg.norm(st))) > 1e-3
+
+
+@pytest.mark.parametrize("theta", [pi / 4, -pi / 4, 5 * pi / 4, 3 * pi / 4])
+def test_cuda_conditional_rotation_ties(theta):
+    """cr at exact quarter-turn ties: rint (half-even) turn counts, gates.cu:366."""
+    nf = 3
+    s, sep = _sim([1, 10], [["setUniform", 0], ["setFock", 1, nf]], conv="cuda")
+    assert s._turns(theta / (pi / 2), "cr") == int(np.rint(theta / (pi / 2)))
+    s.cr(1, 0, 0, theta)
+    s.rz(0, 0, theta * -(2 * nf + 1))
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("theta", [pi / 4, 5 * pi / 4])
+def test_cuda_rotation_ties(theta):
+    """r at exact ties in the cuda convention (gates.cu:327): R(theta)|n> = e^{-i n theta}|n>
+    up to a global phase, so the Fock overlap is 1."""
+    nf = 4
+    s, sep = _sim([10], [["setFock", 0, nf]], conv="cuda")
+    s.r(0, theta)
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
+@pytest.mark.parametrize("theta", [pi / 2, -pi / 2, 3 * pi / 2])
+def test_cuda_beam_splitter_ties(theta):
+    """bs at exact half-turn ties: round half away from zero (gates.cu:476), which
+    differs from TorchCvdv's floor(x + 1/2) at -1/2; binomial law either way."""
+    nf = 4
+    s, _ = _sim([10, 10], [["setFock", 0, nf], ["setFock", 1, 0]], conv="cuda")
+    ratio = theta / pi
+    assert s._turns(ratio, "bs") == int(np.sign(ratio) * np.floor(abs(ratio) + 0.5))
+    s.bs(0, 1, theta)
+    p1, p2 = np.cos(theta / 2) ** 2, np.sin(theta / 2) ** 2
+    th = p2 ** nf
+    for j in range(nf + 1):
+        if j > 0:
+            th *= (nf - j + 1) / j * p1 / p2
+        f = s.getFidelity(_sep([10, 10], [["setFock", 0, j], ["setFock", 1, nf - j]]))
+        assert abs(f - th) < 1e-10
+
+
+@pytest.mark.parametrize("theta", [pi / 2, -pi / 2])
+def test_cuda_conditional_beam_splitter_ties(theta):
+    """cbs at exact half-turn ties (gates.cu:517) against the CBS identity."""
+    nf = 3
+    s, sep = _sim([1, 10, 10], [["setUniform", 0], ["setFock", 1, nf], ["setFock", 2, 0]], conv="cuda")
+    s.cbs(1, 2, 0, 0, theta)
+    s.cp(2, 0, 0)
+    s.bs(1, 2, -theta)
+    assert abs(s.getFidelity(sep) - 1) < 1e-10
+
+
 def test_wigner_and_husimi_closed_form():
     a = 1.0 + 0.5j
     s, _ = _sim([8], [["setCoherent", 0, a]])

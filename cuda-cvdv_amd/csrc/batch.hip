@@ -43,10 +43,23 @@
 #define BATCH_MAX_THREADS 512
 
 // ---------------------------------------------------------------- device data
+// Polynomial phase terms over one register field, merged on the host: for
+// every sign pattern s of up to three control bits, one cubic
+// c[s][0] m + c[s][1] m^2 + c[s][2] m^3 (revolutions) that sums the group's
+// terms with their signs (a term whose control bit is set enters negated).
+// An element then evaluates one cubic per field instead of one per term.
+#define BATCH_GROUP_CTL 3
+#define BATCH_MAX_GROUPS 2
+struct BGroup {
+    int sh, nb, nctl, pad;
+    int ctl[4];
+    double c[1 << BATCH_GROUP_CTL][3];
+};
+
 struct BOp {
     int kind, sh, nb, bit;
     int ctrlFlip, tgtRefl, inv, tfirst;
-    int tn, pad;
+    int tn, gfirst, gn, pad;   // load program: generic terms [tfirst, +tn), groups [gfirst, +gn)
     double grev;      // global phase of the op's load program, revolutions
     double outScale;
     double2 m[4];
@@ -59,6 +72,7 @@ struct BCirc {
     uint32_t initOff;  // c128 entry of register 0's initial array; registers follow in order
     int T, nreg, lds;
     int ftFirst, ftN;  // trailing phase program, applied at the final store
+    int fgFirst, fgN;
     double fgrev;
     uint8_t nq[BATCH_MAX_REG], flw[BATCH_MAX_REG], psh[BATCH_MAX_REG], refl[BATCH_MAX_REG];
 };
@@ -68,6 +82,7 @@ struct BatchArgs {
     const int* order;
     const BOp* ops;
     const DiagTerm* terms;
+    const BGroup* groups;
     const double2* init;
     const void* tw;
     void* out;
@@ -76,11 +91,28 @@ struct BatchArgs {
 };
 
 // ---------------------------------------------------------------- device helpers
+// An op's load program as staged in LDS: merged polynomial groups, the other
+// (generic) terms, the global phase.
+struct SProg {
+    const DiagTerm* t;
+    const BGroup* gr;
+    int tn, gn;
+    double grev;
+    __device__ bool active() const { return tn > 0 || gn > 0 || grev != 0.0; }
+};
+
 template <typename C>
-__device__ __forceinline__ C prog_apply(const DiagTerm* sT, int tn, double grev, uint32_t g, C v) {
+__device__ __forceinline__ C prog_apply(const SProg& P, uint32_t g, C v) {
     using R = decltype(v.x);
-    double r = grev;
-    for (int i = 0; i < tn; ++i) r += cvdv_frac(diag_term_rev(sT[i], g));
+    double r = P.grev;
+    for (int k = 0; k < P.gn; ++k) {
+        const BGroup& G = P.gr[k];
+        int s = 0;
+        for (int j = 0; j < G.nctl; ++j) s |= (int)((g >> G.ctl[j]) & 1u) << j;
+        const double m = (double)((int)(2u * ((g >> G.sh) & ((1u << G.nb) - 1u))) + 1 - (1 << G.nb));
+        r += cvdv_frac(fma(fma(G.c[s][2], m, G.c[s][1]), m, G.c[s][0]) * m);
+    }
+    for (int i = 0; i < P.tn; ++i) r += cvdv_frac(diag_term_rev(P.t[i], g));
     return cmul(v, unit_rev_fast(cvdv_frac(r), R(0)));
 }
 
@@ -101,7 +133,7 @@ __device__ __forceinline__ float2 scale_by(float2 v, double, float hi, float lo)
 template <int R, typename C, typename S>
 __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t rho0, int lgL, const C* tw,
                                        bool first, bool last, bool inv, double sc, float schi, float sclo,
-                                       int tid, int nt) {
+                                       const SProg& P, bool usePre, int tid, int nt) {
     constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
     constexpr int NBF = BATCH_EMAX / R;
     const int lgQ = nb - LGR;
@@ -120,6 +152,7 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
             for (int t = 0; t < R; ++t) {
                 const uint32_t g = base | ((b + ((uint32_t)t << lgQ)) << sh);
                 C v = st[g];
+                if (first && usePre) v = prog_apply(P, g, v);  // the load program, on the first load
                 if (first && inv) v.y = -v.y;  // inverse = conj . forward . conj
                 x[k][t] = v;
             }
@@ -160,17 +193,14 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
 // elements; rows are independent, so chunks need no synchronisation between
 // them beyond the per-stage barriers.
 template <typename C, typename S>
-__device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, const DiagTerm* sT, int tid,
+__device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, const SProg& P, int tid,
                                      int nt, int lgEN) {
     const int nb = op.nb, sh = op.sh;
     const int lgRows = T - nb;
     const int lgC = min(lgRows, lgEN - nb);
     const uint32_t nChunks = 1u << (lgRows - lgC);
-    // the load program first, as its own in-place pass (one element per thread)
-    if (op.tn > 0 || op.grev != 0.0) {
-        for (uint32_t g = tid; g < (1u << T); g += nt) st[g] = prog_apply(sT, op.tn, op.grev, g, st[g]);
-        __syncthreads();
-    }
+    // the load program is applied by the first stage as it loads each element
+    const bool usePre = P.active();
     const float schi = (float)op.outScale;
     const float sclo = (float)(op.outScale - (double)schi);
     for (uint32_t ch = 0; ch < nChunks; ++ch) {
@@ -181,13 +211,13 @@ __device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, co
             const bool first = lgL == 0, last = rem == r;
             if (r == 3)
                 bstage<8>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          tid, nt);
+                          P, usePre, tid, nt);
             else if (r == 2)
                 bstage<4>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          tid, nt);
+                          P, usePre, tid, nt);
             else
                 bstage<2>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          tid, nt);
+                          P, usePre, tid, nt);
             off += (1 << lgL) * ((1 << r) - 1);
             lgL += r;
             rem -= r;
@@ -198,16 +228,16 @@ __device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, co
 // Qubit gate and conditional parity: the same pair maps and FP64 matrix
 // arithmetic as k_qubit / k_cparity (gates.hip); each pair belongs to one thread.
 template <typename C, typename S>
-__device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, const DiagTerm* sT, int tid, int nt) {
+__device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, const SProg& P, int tid, int nt) {
     const uint32_t lowmask = (1u << op.bit) - 1u;
-    const bool usePre = op.tn > 0 || op.grev != 0.0;
+    const bool usePre = P.active();
     for (uint32_t p = tid; p < size / 2; p += nt) {
         const uint32_t i0 = ((p & ~lowmask) << 1) | (p & lowmask);
         const uint32_t i1 = i0 | (1u << op.bit);
         C a = st[i0], b = st[i1];
         if (usePre) {
-            a = prog_apply(sT, op.tn, op.grev, i0, a);
-            b = prog_apply(sT, op.tn, op.grev, i1, b);
+            a = prog_apply(P, i0, a);
+            b = prog_apply(P, i1, b);
         }
         const double2 ad = make_double2((double)a.x, (double)a.y);
         const double2 bd = make_double2((double)b.x, (double)b.y);
@@ -219,18 +249,18 @@ __device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, const
 }
 
 template <typename C, typename S>
-__device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, const DiagTerm* sT, int tid, int nt) {
+__device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, const SProg& P, int tid, int nt) {
     const int top = op.sh + op.nb - 1;
     const uint32_t lowmask = (1u << top) - 1u;
     const uint32_t fmask = ((1u << op.nb) - 1u) << op.sh;
-    const bool usePre = op.tn > 0 || op.grev != 0.0;
+    const bool usePre = P.active();
     for (uint32_t p = tid; p < size / 2; p += nt) {
         const uint32_t i0 = ((p & ~lowmask) << 1) | (p & lowmask);
         const uint32_t i1 = i0 ^ fmask;
         C a = st[i0], b = st[i1];
         if (usePre) {
-            a = prog_apply(sT, op.tn, op.grev, i0, a);
-            b = prog_apply(sT, op.tn, op.grev, i1, b);
+            a = prog_apply(P, i0, a);
+            b = prog_apply(P, i1, b);
         }
         const uint32_t lower = op.tgtRefl ? i1 : i0;
         const int cb = (int)((lower >> op.bit) & 1u) ^ op.ctrlFlip;
@@ -245,17 +275,21 @@ __device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, con
 }
 
 template <typename C, typename S>
-__device__ __forceinline__ void bdiag(S st, uint32_t size, const BOp& op, const DiagTerm* sT, int tid, int nt) {
-    for (uint32_t g = tid; g < size; g += nt) st[g] = prog_apply(sT, op.tn, op.grev, g, st[g]);
+__device__ __forceinline__ void bdiag(S st, uint32_t size, const SProg& P, int tid, int nt) {
+    for (uint32_t g = tid; g < size; g += nt) st[g] = prog_apply(P, g, st[g]);
 }
 
-__device__ __forceinline__ void load_terms(DiagTerm* sT, const DiagTerm* src, int n, int tid) {
-    if (tid < n) sT[tid] = src[tid];
+// Stage a program's generic terms and groups in LDS (one entry per thread).
+__device__ __forceinline__ SProg load_prog(DiagTerm* sT, BGroup* sG, const BatchArgs& a, int tfirst, int tn,
+                                           int gfirst, int gn, double grev, int tid) {
+    if (tid < tn) sT[tid] = a.terms[tfirst + tid];
+    if (tid >= CVDV_MAX_TERMS && tid < CVDV_MAX_TERMS + gn) sG[tid - CVDV_MAX_TERMS] = a.groups[gfirst + tid - CVDV_MAX_TERMS];
+    return SProg{sT, sG, tn, gn, grev};
 }
 
 template <typename C, typename S>
-__device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchArgs& a, DiagTerm* sT, int tid,
-                                            int nt, int lgEN) {
+__device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchArgs& a, DiagTerm* sT, BGroup* sG,
+                                            int tid, int nt, int lgEN) {
     const uint32_t size = 1u << cd.T;
     // 1. product state in the canonical layout (k_init_product order: register 0 first)
     {
@@ -276,30 +310,26 @@ __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchAr
     // 2. ops
     for (uint32_t i = cd.opBegin; i < cd.opEnd; ++i) {
         const BOp& op = a.ops[i];
-        if (op.tn > 0) {
-            load_terms(sT, a.terms + op.tfirst, op.tn, tid);
-            __syncthreads();
-        }
+        const SProg P = load_prog(sT, sG, a, op.tfirst, op.tn, op.gfirst, op.gn, op.grev, tid);
+        if (op.tn > 0 || op.gn > 0) __syncthreads();
         switch (op.kind) {
             case BOP_FFT:
-                bfft<C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], sT, tid, nt, lgEN);
+                bfft<C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN);
                 break;
-            case BOP_QUBIT: bqubit<C>(st, size, op, sT, tid, nt); break;
-            case BOP_CPARITY: bcparity<C>(st, size, op, sT, tid, nt); break;
-            default: bdiag<C>(st, size, op, sT, tid, nt); break;
+            case BOP_QUBIT: bqubit<C>(st, size, op, P, tid, nt); break;
+            case BOP_CPARITY: bcparity<C>(st, size, op, P, tid, nt); break;
+            default: bdiag<C>(st, size, P, tid, nt); break;
         }
         __syncthreads();
     }
     // 3. canonical store with the trailing program
-    if (cd.ftN > 0) {
-        load_terms(sT, a.terms + cd.ftFirst, cd.ftN, tid);
-        __syncthreads();
-    }
-    const bool useFinal = cd.ftN > 0 || cd.fgrev != 0.0;
+    const SProg F = load_prog(sT, sG, a, cd.ftFirst, cd.ftN, cd.fgFirst, cd.fgN, cd.fgrev, tid);
+    if (cd.ftN > 0 || cd.fgN > 0) __syncthreads();
+    const bool useFinal = F.active();
     C* out = (C*)a.out + cd.outOff;
     for (uint32_t g = tid; g < size; g += nt) {
         C v = st[g];
-        if (useFinal) v = prog_apply(sT, cd.ftN, cd.fgrev, g, v);
+        if (useFinal) v = prog_apply(F, g, v);
         uint32_t o = 0;
         for (int r = 0; r < cd.nreg; ++r) {
             const uint32_t mask = (1u << cd.nq[r]) - 1u;
@@ -315,12 +345,13 @@ template <typename C>
 __global__ __launch_bounds__(BATCH_MAX_THREADS) void k_batch(BatchArgs a, int lgEN) {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ DiagTerm sT[CVDV_MAX_TERMS];
+    __shared__ BGroup sG[BATCH_MAX_GROUPS];
     const BCirc& cd = a.circ[a.order[blockIdx.x]];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (cd.lds)
-        run_circuit<C>((C*)smem, cd, a, sT, tid, nt, lgEN);
+        run_circuit<C>((C*)smem, cd, a, sT, sG, tid, nt, lgEN);
     else
-        run_circuit<C>((C*)a.work + cd.workOff, cd, a, sT, tid, nt, lgEN);
+        run_circuit<C>((C*)a.work + cd.workOff, cd, a, sT, sG, tid, nt, lgEN);
 }
 
 // ---------------------------------------------------------------- readout kernel
@@ -404,6 +435,8 @@ struct CVDVBatch {
     size_t opsCap = 0;
     DiagTerm* dTerms = nullptr;
     size_t termsCap = 0;
+    BGroup* dGroups = nullptr;
+    size_t groupsCap = 0;
     void* dTw = nullptr;
     int twOff[BATCH_MAX_REG_QUBITS + 1] = {0};
     double* dRes = nullptr;
@@ -490,10 +523,65 @@ static void batch_compile(CVDVBatch* b) {
     std::vector<double> cost(b->n);
     size_t workElems = 0;
     uint32_t initOff = 0;
-    auto add_terms = [&](const DiagProg& p, int& first, int& n) {
+    std::vector<BGroup> hgroups;
+    // A program's polynomial terms merge into at most BATCH_MAX_GROUPS groups (one
+    // per register field, up to BATCH_GROUP_CTL control bits each); the rest stay
+    // generic terms.
+    auto add_terms = [&](const DiagProg& p, int& first, int& n, int& gfirst, int& gn) {
         first = (int)hterms.size();
-        n = p.n;
-        for (int t = 0; t < p.n; ++t) hterms.push_back(p.t[t]);
+        gfirst = (int)hgroups.size();
+        n = 0;
+        gn = 0;
+        std::vector<BGroup> gs;
+        std::vector<std::vector<const DiagTerm*>> mem;
+        for (int t = 0; t < p.n; ++t) {
+            const DiagTerm& T = p.t[t];
+            if (T.c1 == 0.0 && T.c2 == 0.0 && T.c3 == 0.0) continue;
+            int k = -1;
+            if (T.kind == DT_POLY) {
+                for (int j = 0; j < (int)gs.size(); ++j)
+                    if (gs[j].sh == T.sh1 && gs[j].nb == T.nb1) k = j;
+                if (k < 0 && (int)gs.size() < BATCH_MAX_GROUPS) {
+                    BGroup G{};
+                    G.sh = T.sh1;
+                    G.nb = T.nb1;
+                    gs.push_back(G);
+                    mem.emplace_back();
+                    k = (int)gs.size() - 1;
+                }
+                if (k >= 0 && T.ctrl >= 0) {
+                    bool have = false;
+                    for (int j = 0; j < gs[k].nctl; ++j) have |= gs[k].ctl[j] == T.ctrl;
+                    if (!have) {
+                        if (gs[k].nctl == BATCH_GROUP_CTL) k = -1;
+                        else gs[k].ctl[gs[k].nctl++] = T.ctrl;
+                    }
+                }
+            }
+            if (k >= 0) {
+                mem[k].push_back(&T);
+            } else {
+                hterms.push_back(T);
+                ++n;
+            }
+        }
+        for (size_t k = 0; k < gs.size(); ++k) {
+            BGroup& G = gs[k];
+            for (int sgn = 0; sgn < (1 << BATCH_GROUP_CTL); ++sgn) {
+                double c[3] = {0.0, 0.0, 0.0};
+                for (const DiagTerm* T : mem[k]) {
+                    double f = 1.0;
+                    for (int j = 0; j < G.nctl; ++j)
+                        if (T->ctrl == G.ctl[j] && ((sgn >> j) & 1)) f = -1.0;
+                    c[0] += f * T->c1;
+                    c[1] += f * T->c2;
+                    c[2] += f * T->c3;
+                }
+                for (int q = 0; q < 3; ++q) G.c[sgn][q] = c[q];
+            }
+            hgroups.push_back(G);
+            ++gn;
+        }
     };
     for (int i = 0; i < b->n; ++i) {
         const CVDVContext* c = b->circ[i];
@@ -531,14 +619,14 @@ static void batch_compile(CVDVBatch* b) {
             for (int k = 0; k < 4; ++k) o.m[k] = op.m[k];
             if (op.prog >= 0) {
                 const DiagProg& p = c->rec->progs[op.prog];
-                add_terms(p, o.tfirst, o.tn);
+                add_terms(p, o.tfirst, o.tn, o.gfirst, o.gn);
                 o.grev = p.grev;
             }
             hops.push_back(o);
             cst += op.kind == BOP_FFT ? 1.0 + (op.nb + 2) / 3 : 1.0;
         }
         d.opEnd = (uint32_t)hops.size();
-        add_terms(c->pending, d.ftFirst, d.ftN);
+        add_terms(c->pending, d.ftFirst, d.ftN, d.fgFirst, d.fgN);
         d.fgrev = c->pending.grev;
         // global-mode circuits pay cache traffic instead of LDS traffic
         cost[i] = cst * (double)size * (d.lds ? 1.0 : 2.0);
@@ -570,6 +658,9 @@ static void batch_compile(CVDVBatch* b) {
     bgrow(b->dTerms, b->termsCap, hterms.size());
     if (!hterms.empty())
         HIPCHK(hipMemcpy(b->dTerms, hterms.data(), hterms.size() * sizeof(DiagTerm), hipMemcpyHostToDevice));
+    bgrow(b->dGroups, b->groupsCap, hgroups.size());
+    if (!hgroups.empty())
+        HIPCHK(hipMemcpy(b->dGroups, hgroups.data(), hgroups.size() * sizeof(BGroup), hipMemcpyHostToDevice));
     if (workElems > b->workElems || !b->dWork) {
         bfree(b->dWork);
         HIPCHK(hipMalloc(&b->dWork, std::max<size_t>(workElems, 1) * eb));
@@ -583,6 +674,7 @@ static void batch_compile(CVDVBatch* b) {
     a.order = b->dOrder;
     a.ops = b->dOps;
     a.terms = b->dTerms;
+    a.groups = b->dGroups;
     a.init = b->dInit;
     a.tw = b->dTw;
     a.out = b->dOut;
@@ -633,6 +725,7 @@ static void batch_release(CVDVBatch* b) {
     bfree((void*&)b->dOrder);
     bfree((void*&)b->dOps);
     bfree((void*&)b->dTerms);
+    bfree((void*&)b->dGroups);
     bfree(b->dTw);
     bfree((void*&)b->dRes);
     bfree((void*&)b->dBinOff);

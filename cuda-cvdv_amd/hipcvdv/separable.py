@@ -38,23 +38,33 @@ class SeparableState:
         arr[0] = 1.0
         self.register_arrays[regIdx] = arr
 
-    def setCoherent(self, regIdx: int, alpha: Union[complex, float, int]) -> None:
-        """Coherent state |alpha>: pi^-1/4 sqrt(dx) e^{-(x-q)^2/2} e^{i(p x - p q/2)}."""
-        alpha = complex(alpha)
+    def _coherent(self, regIdx: int, alpha: complex) -> torch.Tensor:
+        """pi^-1/4 sqrt(dx) e^{-(x-q)^2/2} e^{i(p x - p q/2)}, q = sqrt2 Re a, p = sqrt2 Im a."""
         x = self._x(regIdx)
-        q = 2 ** 0.5 * alpha.real
-        p = 2 ** 0.5 * alpha.imag
-        amp = _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5 * torch.exp(-0.5 * (x - q) ** 2)
-        self.register_arrays[regIdx] = amp * torch.exp(1j * (p * x - p * q / 2.0)).to(torch.complex128)
+        q, p = 2 ** 0.5 * alpha.real, 2 ** 0.5 * alpha.imag
+        envelope = _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5 * torch.exp(-0.5 * (x - q) ** 2)
+        return envelope * torch.exp(1j * (p * x - p * q / 2.0)).to(torch.complex128)
+
+    def setCoherent(self, regIdx: int, alpha: Union[complex, float, int]) -> None:
+        """Coherent state |alpha> on the grid (not renormalised)."""
+        self.register_arrays[regIdx] = self._coherent(regIdx, complex(alpha))
+
+    def _hermite(self, regIdx: int, count: int) -> torch.Tensor:
+        """(count, N) real table of the grid-sampled Hermite functions
+        h_n(x) pi^-1/4 sqrt(dx) e^{-x^2/2} (discrete norm ~1), n < count, by
+        h_n = sqrt(2/n) x h_{n-1} - sqrt((n-1)/n) h_{n-2}."""
+        x = self._x(regIdx)
+        table = torch.empty((max(count, 1), x.numel()), dtype=torch.float64, device=self.device)
+        table[0] = torch.exp(-0.5 * x * x) * _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5
+        lower = torch.zeros_like(x)
+        for n in range(1, count):
+            table[n] = (2.0 / n) ** 0.5 * x * table[n - 1] - ((n - 1.0) / n) ** 0.5 * lower
+            lower = table[n - 1]
+        return table
 
     def setFock(self, regIdx: int, n: int) -> None:
-        """Fock state |n> by the Hermite-function three-term recurrence (not renormalised)."""
-        x = self._x(regIdx)
-        prev = torch.zeros_like(x)
-        cur = torch.exp(-0.5 * x * x) * _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5
-        for k in range(1, n + 1):
-            prev, cur = cur, (2.0 / k) ** 0.5 * x * cur - ((k - 1.0) / k) ** 0.5 * prev
-        self.register_arrays[regIdx] = cur.to(torch.complex128)
+        """Fock state |n> (grid-sampled Hermite function; not renormalised)."""
+        self.register_arrays[regIdx] = self._hermite(regIdx, n + 1)[n].to(torch.complex128)
 
     def setUniform(self, regIdx: int) -> None:
         """Uniform superposition N^-1/2 sum_k |k>."""
@@ -66,18 +76,11 @@ class SeparableState:
         """sum_n c_n |n> (not renormalised); all-zero coefficients give setZero."""
         c = torch.as_tensor(coeffs if isinstance(coeffs, torch.Tensor) else list(coeffs),
                             dtype=torch.complex128).to(self.device)
-        if not bool(torch.any(torch.abs(c) > 1e-12)):
+        if not bool((c.abs() > 1e-12).any()):
             self.setZero(regIdx)
             return
-        x = self._x(regIdx)
-        nmax = c.numel()
-        basis = torch.zeros((nmax, x.numel()), dtype=torch.float64, device=self.device)
-        basis[0] = torch.exp(-0.5 * x * x) * _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5
-        if nmax > 1:
-            basis[1] = 2.0 ** 0.5 * x * basis[0]
-            for k in range(2, nmax):
-                basis[k] = (2.0 / k) ** 0.5 * x * basis[k - 1] - ((k - 1.0) / k) ** 0.5 * basis[k - 2]
-        self.register_arrays[regIdx] = torch.sum(c[:, None] * basis.to(torch.complex128), dim=0)
+        table = self._hermite(regIdx, c.numel()).to(torch.complex128)
+        self.register_arrays[regIdx] = (c.unsqueeze(1) * table).sum(dim=0)
 
     def setCoeffs(self, regIdx: int, coeffs: Union[Sequence[complex], torch.Tensor]) -> None:
         """Arbitrary position-basis amplitudes (used as given)."""
@@ -92,21 +95,16 @@ class SeparableState:
 
     def setCat(self, regIdx: int,
                cat_states: Sequence[Tuple[Union[complex, float], Union[complex, float]]]) -> None:
-        """Normalised superposition sum_k c_k |alpha_k> of coherent states."""
-        x = self._x(regIdx)
-        alphas = [complex(a) for a, _ in cat_states]
-        coeffs = torch.tensor([complex(c) for _, c in cat_states], dtype=torch.complex128,
-                              device=self.device)
-        q = torch.tensor([2 ** 0.5 * a.real for a in alphas], dtype=torch.float64, device=self.device)
-        p = torch.tensor([2 ** 0.5 * a.imag for a in alphas], dtype=torch.float64, device=self.device)
-        gauss = torch.exp(-0.5 * (x[None, :] - q[:, None]) ** 2)
-        phase = p[:, None] * x[None, :] - (p * q)[:, None] / 2.0
-        coh = _PI_M_QUARTER * self.grid_steps[regIdx] ** 0.5 * gauss * torch.exp(1j * phase).to(torch.complex128)
-        state = torch.sum(coeffs[:, None] * coh, dim=0)
-        self.register_arrays[regIdx] = state / torch.sqrt(torch.sum(torch.abs(state) ** 2))
+        """sum_k c_k |alpha_k>, scaled to unit norm over the grid."""
+        parts = [(complex(c), self._coherent(regIdx, complex(a))) for a, c in cat_states]
+        weights = torch.tensor([c for c, _ in parts], dtype=torch.complex128, device=self.device)
+        waves = torch.stack([w for _, w in parts])
+        psi = (weights.unsqueeze(1) * waves).sum(dim=0)
+        self.register_arrays[regIdx] = psi / psi.abs().square().sum().sqrt()
 
     def validate(self) -> None:
-        """Raise if any register has not been initialised."""
-        for i, arr in enumerate(self.register_arrays):
-            if arr is None:
-                raise RuntimeError(f"Register {i} not initialized in SeparableState")
+        """Raise RuntimeError naming the first register that no setter has filled."""
+        missing = [r for r, a in enumerate(self.register_arrays) if a is None]
+        if missing:
+            raise RuntimeError(f"SeparableState: register {missing[0]} has no wavefunction "
+                               f"(call one of the set* methods for it)")

@@ -308,10 +308,19 @@ def c3_section(steps):
         t0 = time.perf_counter()
         jp = h.m(*range(nq))
         t_joint = time.perf_counter() - t0
+        ct = {}
+        try:
+            _ct_marginal(h, list(range(nq)))
+            t0 = time.perf_counter()
+            jct = _ct_marginal(h, list(range(nq)))
+            ct = {"hiptensor_joint_ms": (time.perf_counter() - t0) * 1e3,
+                  "hiptensor_joint_max_abs_diff": float(np.max(np.abs(jct - np.asarray(jp))))}
+        except Exception as e:  # the library may not reduce a 15-mode tensor
+            ct = {"hiptensor_joint": f"unsupported: {e}"}
         joint[prec] = (np.asarray(jp), [np.asarray(x) for x in singles])
         res[prec] = {"circuit_ms": t_circ * 1e3, "single_marginals_ms": t_single * 1e3,
                      "joint_marginal_ms": t_joint * 1e3, "joint_bins": int(np.asarray(jp).size),
-                     "prob_sum": float(np.asarray(jp).sum())}
+                     "prob_sum": float(np.asarray(jp).sum()), **ct}
         print(f"[bench] c3 {prec}: circuit {t_circ * 1e3:.1f} ms, 14 marginals {t_single * 1e3:.1f} ms, "
               f"joint {t_joint * 1e3:.1f} ms", file=sys.stderr, flush=True)
         del h
@@ -367,25 +376,61 @@ def c5_section(steps):
                          ("husimi_512_per_branch", lambda: h.getHusimiQGrid(2, grid, grid, per_slice=True)),
                          ("wigner_native_traced", lambda: h.getWigner(2)),
                          ("husimi_native_traced", lambda: h.getHusimiQ(2)),
-                         ("joint_m012", lambda: h.m(0, 1, 2))):
+                         # the same readouts into device memory (no 128 MiB copy to pageable host memory)
+                         ("wigner_native_traced_device", lambda: h.getWigner(2, device=True).cpu()),
+                         ("husimi_native_traced_device", lambda: h.getHusimiQ(2, device=True).cpu()),
+                         ("joint_m012", lambda: h.m(0, 1, 2)),
+                         ("joint_m012_hiptensor", lambda: _ct_marginal(h, [0, 1, 2]))):
             fn()  # untimed: first-call costs (code objects, library handles, work buffers)
             t0 = time.perf_counter()
             for _ in range(steps):
                 val = fn()
             t[name + "_ms"] = (time.perf_counter() - t0) / steps * 1e3
             outs[(prec, name)] = np.asarray(val)
+        # device readouts: the time spent in the library's kernels, for the host-side share
+        h._lib.cvdvProfileEnable(h.ctx, 1)
+        h.getWigner(2, device=True)
+        t["wigner_native_kernels_ms"] = _prof_total(h)
+        h._lib.cvdvProfileReset(h.ctx)
+        h.getHusimiQ(2, device=True)
+        t["husimi_native_kernels_ms"] = _prof_total(h)
+        h._lib.cvdvProfileEnable(h.ctx, 0)
         res[prec] = t
         print(f"[bench] c5 {prec}: " + ", ".join(f"{k} {v:.1f}" for k, v in t.items()), file=sys.stderr, flush=True)
         del h
         torch.cuda.empty_cache()
     err = {}
     for name in ("wigner_512_per_branch", "husimi_512_per_branch", "wigner_native_traced", "husimi_native_traced",
-                 "joint_m012"):
+                 "joint_m012", "joint_m012_hiptensor"):
         a, b = outs[("mixed", name)], outs[("complex128", name)]
         err[name] = {"max_abs_err": float(np.max(np.abs(a - b))), "max_abs_fp64": float(np.max(np.abs(b)))}
     return {"workload": "[2,2,12], 16 qubit branches, 20 seeded cd/cr; per-branch Wigner and Husimi on "
                         "512x512 over [-6,6]^2, native traced readouts, joint m(0,1,2)",
             "mixed": res["mixed"], "complex128": res["complex128"], "mixed_vs_fp64": err}
+
+
+def _ct_marginal(h, regs):
+    """cvdvMeasureMultipleCT (the hipTensor comparator), shaped like HipCvdv.m()."""
+    import ctypes
+
+    from hipcvdv import _lib
+
+    dims = [h.register_dims[r] for r in regs]
+    out = np.zeros(int(np.prod(dims)))
+    sel = (ctypes.c_int * len(regs))(*regs)
+    h._lib.cvdvMeasureMultipleCT(h.ctx, sel, len(regs), out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    _lib.check()
+    return out.reshape(dims[::-1]).T if len(dims) > 1 else out
+
+
+def _prof_total(h):
+    """Device time of the readouts recorded on the context: HIP events around each
+    phase-space readout's kernels (profile kind 5), milliseconds."""
+    from ctypes import byref, c_double
+
+    tot, mx = c_double(0), c_double(0)
+    h._lib.cvdvProfileRead(h.ctx, 5, byref(tot), byref(mx))
+    return tot.value
 
 
 def c1_section(steps):
@@ -433,11 +478,17 @@ def c1_section(steps):
             "max_abs_err_m0": float(np.max(np.abs(np.asarray(m0) - np.asarray(mo))))}
 
 
-def run_gpu(prec, ops, steps, warmup):
+def run_gpu(prec, ops, steps, warmup, fuse=True):
+    """Time `steps` runs of the circuit on one context.  The timed region ends with
+    cvdvFlush, which completes a half-done transform, applies the pending phases
+    and restores the canonical layout: every gate of every step has executed."""
+    import ctypes
+
     import torch
     from hipcvdv import HipCvdv, SeparableState
 
     h = HipCvdv(REGS, precision=prec)
+    h._c("cvdvSetTransformFusion", ctypes.c_int(1 if fuse else 0))
     sep = SeparableState(REGS)
     sep.setCoherent(0, 1.2 - 0.4j)
     sep.setCoeffs(1, torch.from_numpy(squeezed_vacuum(REGS[1], 0.5)))
@@ -449,23 +500,29 @@ def run_gpu(prec, ops, steps, warmup):
 
     for _ in range(warmup):
         step()
+    h.flush()
     h.synchronize()
     torch.cuda.synchronize()
     h._lib.cvdvProfileEnable(h.ctx, 1)
     t0 = time.perf_counter()
     for _ in range(steps):
         step()
+    h.flush()
     h.synchronize()
     torch.cuda.synchronize()
     el = time.perf_counter() - t0
     from ctypes import byref, c_double
     tot, mx = c_double(0), c_double(0)
-    n_fft = h._lib.cvdvProfileRead(h.ctx, 0, byref(tot), byref(mx))
-    prof = {"fft_launches": n_fft, "fft_ms": tot.value}
-    for kind, name in ((1, "diag"), (2, "qubit"), (4, "reduce"), (5, "other")):
+    prof = {}
+    for kind, name in ((0, "fft"), (6, "fft_fused"), (7, "fft_row"), (1, "diag"), (2, "qubit"), (3, "cparity"),
+                       (4, "reduce"), (5, "other")):
         n = h._lib.cvdvProfileRead(h.ctx, kind, byref(tot), byref(mx))
         prof[name + "_launches"] = n
         prof[name + "_ms"] = tot.value
+    # the remaining FFT passes: four-step halves that do not fuse (first halves,
+    # completions) and single-pass strided tiles
+    prof["fft_half_launches"] = prof["fft_launches"] - prof["fft_fused_launches"] - prof["fft_row_launches"]
+    prof["fft_half_ms"] = prof["fft_ms"] - prof["fft_fused_ms"] - prof["fft_row_ms"]
     h._lib.cvdvProfileEnable(h.ctx, 0)
     return h, el, prof
 
@@ -625,10 +682,91 @@ def dft_gemm_probe(reps=5):
     return out
 
 
-def pmc_traffic():
-    """HBM bytes per FFT launch from the committed counter summary (FETCH_SIZE and
-    WRITE_SIZE, each scaled by the same counter on a plain copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "k_fft_tile_c2_mixed_pmc.txt")
+def _fock_table(N):
+    """Grid-sampled Hermite functions |n>, n < N, by the three-term recurrence
+    (separable.py:62-74's builder for every n at once), FP64."""
+    dx = math.sqrt(2 * math.pi / N)
+    x = (np.arange(N) - (N - 1) / 2) * dx
+    F = np.empty((N, N))
+    F[0] = math.pi ** -0.25 * math.sqrt(dx) * np.exp(-0.5 * x * x)
+    prev = np.zeros(N)
+    for k in range(1, N):
+        F[k] = math.sqrt(2.0 / k) * x * F[k - 1] - math.sqrt((k - 1.0) / k) * prev
+        prev = F[k - 1]
+    return F
+
+
+def _upper_fit(ks, eps):
+    """log eps = a k + b by least squares, b raised by the largest residual so the
+    line bounds every point (the reference's fit_ab_upper, analysis/_common.py:67-86)."""
+    if len(ks) < 2:
+        return None
+    y = np.log(np.asarray(eps))
+    a, b = np.polyfit(np.asarray(ks, dtype=float), y, 1)
+    b += float(np.max(y - (a * np.asarray(ks) + b)))
+    return float(a), float(b)
+
+
+def qft_error_envelope(qubits=range(6, 13)):
+    """Precision envelope of the register QFT (the reference's analysis/qft_err.py
+    and ERROR-ANALYSIS.md:13-16, rerun on this library): for N = 2^q, every Fock
+    state |n> is transformed by ftQ2P (all n at once, as the slices of an [q, q]
+    state) and compared with the analytic (-i)^n |n>; eps_n is the L2 error.  Per
+    precision and N: the reference's upper-bound fit log eps <= a k + b over the
+    points with eps >= 1e-9 (complex128) or >= 10x the mixed rounding floor, up to
+    the first n with eps^2 >= 0.1; the floor (median eps over n < 8) and the
+    largest error of mixed against complex128 on the same inputs.  Then
+    a(N) = c_a N^-1/2 + d_a and b(N) = c_b N^1/2 + d_b, as in the reference."""
+    import torch
+    from hipcvdv import HipCvdv
+
+    out = {"per_N": [], "fits": {}}
+    fits = {"complex128": [], "mixed": []}
+    for q in qubits:
+        N = 1 << q
+        F = _fock_table(N)
+        target = F * ((-1j) ** (np.arange(N) % 4))[:, None]
+        res = {}
+        for prec in ("complex128", "mixed"):
+            h = HipCvdv([q, q], precision=prec)
+            h.initFromArray(F.astype(np.complex128).reshape(-1))
+            h.ftQ2P(1)
+            got = h.getState().reshape(N, N)
+            del h
+            torch.cuda.empty_cache()
+            res[prec] = got
+        row = {"N": N}
+        for prec, got in res.items():
+            eps = np.sqrt(np.sum(np.abs(got - target) ** 2, axis=1))
+            stop = int(np.argmax(eps ** 2 >= 0.1)) if np.any(eps ** 2 >= 0.1) else N
+            floor = float(np.median(eps[:8]))
+            cut = 1e-9 if prec == "complex128" else 10 * floor
+            ks = [k for k in range(stop + 1) if eps[k] >= cut]
+            ab = _upper_fit(ks, eps[ks])
+            row[prec] = {"floor": floor, "n_stop": stop, "fit_points": len(ks),
+                         "a": ab[0] if ab else None, "b": ab[1] if ab else None}
+            if ab:
+                fits[prec].append((N, ab[0], ab[1]))
+        d = np.sqrt(np.sum(np.abs(res["mixed"] - res["complex128"]) ** 2, axis=1))
+        row["mixed_vs_c128_max_l2"] = float(np.max(d[: max(row["complex128"]["n_stop"], 1)]))
+        out["per_N"].append(row)
+        print(f"[bench] qft envelope N={N}: c128 floor {row['complex128']['floor']:.1e}, mixed floor "
+              f"{row['mixed']['floor']:.1e}, mixed-vs-c128 {row['mixed_vs_c128_max_l2']:.1e}", file=sys.stderr,
+              flush=True)
+    for prec, fp in fits.items():
+        if len(fp) >= 2:
+            Ns = np.array([f[0] for f in fp], dtype=float)
+            ca, da = np.polyfit(Ns ** -0.5, [f[1] for f in fp], 1)
+            cb, db = np.polyfit(Ns ** 0.5, [f[2] for f in fp], 1)
+            out["fits"][prec] = {"a(N)": [float(ca), float(da)], "b(N)": [float(cb), float(db)]}
+    return out
+
+
+def pmc_traffic(kernel):
+    """HBM bytes per launch of `kernel` from its committed counter summary
+    (profiles/<kernel>_c2_mixed_pmc.txt: FETCH_SIZE and WRITE_SIZE, each scaled
+    by the same counter on a plain copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_c2_mixed_pmc.txt")
     try:
         with open(path) as f:
             for line in f:
@@ -637,6 +775,38 @@ def pmc_traffic():
     except OSError:
         pass
     return None
+
+
+# FFT pass kinds of the profile (cvdvProfileRead) -> kernel name and the register
+# transforms one launch performs (a fused pass completes one transform and starts
+# the next: one transform's worth; a lone half: half of one; a one-row tile: one)
+FFT_KINDS = {"fft_fused": ("k_fft_fused", 1.0), "fft_row": ("k_fft_tile (one-row)", 1.0),
+             "fft_half": ("k_fft_tile (four-step half)", 0.5)}
+
+
+def roofline(prof, pass_bytes, copy):
+    """Roofline of the FFT pass kind with the largest summed time: algorithmic bytes
+    per launch (transforms per launch x 2 A s) over its mean launch duration, both
+    measured with HIP events on the context's stream in the timed region."""
+    kinds = {k: v for k, v in FFT_KINDS.items() if prof.get(k + "_launches", 0) > 0}
+    if not kinds:
+        return None
+    dom = max(kinds, key=lambda k: prof[k + "_ms"])
+    name, per = FFT_KINDS[dom]
+    n, ms = prof[dom + "_launches"], prof[dom + "_ms"]
+    avg = ms / n
+    alg = per * pass_bytes
+    achieved = alg / (avg * 1e-3) / 1e9
+    out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(name.split()[0]) if dom == "fft_fused" else None,
+           "kernel": name, "launches": n, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": alg,
+           "copy_gbps": copy["gbps"], "frac_of_copy": achieved / copy["gbps"], "share_of_fft_ms": ms / prof["fft_ms"],
+           "fft_kinds": {k: {"kernel": FFT_KINDS[k][0], "launches": prof[k + "_launches"], "ms": prof[k + "_ms"],
+                             "avg_ms": prof[k + "_ms"] / prof[k + "_launches"] if prof[k + "_launches"] else None,
+                             "gbps": (FFT_KINDS[k][1] * pass_bytes / (prof[k + "_ms"] / prof[k + "_launches"] * 1e-3) / 1e9)
+                             if prof[k + "_launches"] else None}
+                         for k in FFT_KINDS}}
+    return out
 
 
 def main():
@@ -658,6 +828,10 @@ def main():
     ap.add_argument("--no-c5", action="store_true", help="skip the phase-space readout workloads (C1, C5)")
     ap.add_argument("--copy-only", action="store_true",
                     help="only the plain-copy bandwidth probe (counter calibration runs)")
+    ap.add_argument("--qft-error", action="store_true",
+                    help="only the QFT precision envelope (Fock states, both precisions)")
+    ap.add_argument("--no-fuse", action="store_true",
+                    help="run every register transform whole (two passes for the strided register)")
     ap.add_argument("--dft-gemm", action="store_true",
                     help="only the DFT-as-GEMM (MFMA) against register-FFT comparison")
     args = ap.parse_args()
@@ -668,6 +842,9 @@ def main():
 
     if args.dft_gemm:
         print(json.dumps({"dft_gemm": dft_gemm_probe()}))
+        return
+    if args.qft_error:
+        print(json.dumps({"qft_error": qft_error_envelope()}))
         return
     if args.copy_only:
         print(json.dumps({"copy": copy_probe()}))
@@ -687,7 +864,7 @@ def main():
     res = {}
     precs = [args.only] if args.only else ["mixed", "complex128"]
     for prec in precs:
-        h, el, prof = run_gpu(prec, ops, args.steps, args.warmup)
+        h, el, prof = run_gpu(prec, ops, args.steps, args.warmup, fuse=not args.no_fuse)
         s = 8 if prec == "mixed" else 16
         res[prec] = {"t": el, "prof": prof, "s": s}
         del h
@@ -700,10 +877,6 @@ def main():
     r = res[main_prec]
     value = passes * args.steps / r["t"]
     pass_bytes = 2 * A * r["s"]
-    fft_alg_bytes = passes * args.steps * pass_bytes
-    fft_launches = r["prof"]["fft_launches"]
-    fft_ms = r["prof"]["fft_ms"]
-    achieved = fft_alg_bytes / (fft_ms * 1e-3) / 1e9 if fft_ms > 0 else None
     copy = copy_probe()
     out = {
         "metric": METRIC,
@@ -721,16 +894,13 @@ def main():
         "config": {"workload": "[14,14] oscillator registers, 2^28 amplitudes, 200 seeded Gaussian gates "
                                "(d, r, s, bs 50:50, bs random), seed 2",
                    "amplitudes": A, "passes_per_step": passes,
+                   "fused_transforms": not args.no_fuse,
+                   "fft_launches_per_step": r["prof"]["fft_launches"] / args.steps,
                    "precision": "complex64 amplitudes; FP64 angles, twiddles, sums" if main_prec == "mixed"
                    else "complex128"},
         "pct_hbm_peak": value * pass_bytes / 1e9 / HBM_PEAK_GBS * 100.0,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": pmc_traffic(),
-                     "copy_gbps": copy["gbps"],
-                     "frac_of_copy": (achieved / copy["gbps"]) if achieved else None,
-                     "kernel": "k_fft_tile", "fft_launches": fft_launches,
-                     "avg_launch_ms": fft_ms / fft_launches if fft_launches else None,
-                     "algorithmic_bytes_per_launch": fft_alg_bytes / fft_launches if fft_launches else None},
+        "roofline": roofline(r["prof"], pass_bytes, copy),
+        "copy_probe": copy,
     }
     if "complex128" in res and "mixed" in res:
         v64 = passes * args.steps / res["complex128"]["t"]

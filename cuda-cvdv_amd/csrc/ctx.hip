@@ -128,6 +128,7 @@ static void ctx_release(CVDVContext* ctx) {
     ctx->tw = nullptr;
     blas_release(ctx);
     ctx->generation++;  // captured graphs of the released buffers become invalid
+    ctx->defer = FtDefers{};
     ctx->initialized = false;
 }
 
@@ -137,6 +138,7 @@ static void reset_layout(CVDVContext* ctx) {
         ctx->refl[r] = 0;
     }
     ctx->pending = DiagProg{};
+    ctx->defer = FtDefers{};  // a new state: nothing half done
     ctx->cur = 0;
 }
 
@@ -263,6 +265,7 @@ static unsigned grid_for(uint64_t total, unsigned block = 256) {
 }
 
 void ctx_materialize(CVDVContext* ctx) {
+    fft_settle(ctx);
     if (ctx->canonical()) {
         ctx_flush(ctx);
         return;
@@ -538,7 +541,8 @@ int cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs) 
     int n = 0;
     double tot = 0.0, mx = 0.0;
     for (auto& r : ctx->prof) {
-        if (r.kind != kind) continue;
+        const bool fft = r.kind == KK_FFT || r.kind == KK_FFT_FUSED || r.kind == KK_FFT_ROW;
+        if (r.kind != kind && !(kind == KK_FFT && fft)) continue;
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, r.a, r.b));
         tot += ms;
@@ -556,6 +560,14 @@ void cvdvProfileReset(CVDVContext* ctx) {
     CVDV_TRY
     if (ctx->stream) HIPCHK(hipStreamSynchronize(ctx->stream));
     prof_clear(ctx);
+    CVDV_CATCH()
+}
+
+void cvdvSetTransformFusion(CVDVContext* ctx, int on) {
+    if (!ctx || refuse_recording(ctx)) return;
+    CVDV_TRY
+    if (!on && ctx->initialized) fft_settle(ctx);
+    ctx->fuseFt = on ? 1 : 0;
     CVDV_CATCH()
 }
 
@@ -638,6 +650,7 @@ CVDVGraph* cvdvGraphEnd(CVDVContext* ctx) {
         ctx->refl[r] = ctx->capSave.refl[r];
     }
     ctx->pending = ctx->capSave.pending;
+    ctx->defer = FtDefers{};  // cvdvGraphBegin settled it
     ctx->cur = ctx->capSave.cur;
     if (!ok || ce != hipSuccess || !g) {
         if (g) (void)hipGraphDestroy(g);

@@ -164,12 +164,52 @@ struct BatchRecording {
 struct TwiddleCache;  // fft.hip
 
 // Kernel kinds for the optional per-launch event timing (cvdvProfileEnable).
+// KK_FFT counts every FFT pass when read; launches are recorded as KK_FFT_FUSED
+// (two half transforms in one pass), KK_FFT_ROW (one whole contiguous row of
+// 2^12 points or more per tile) or KK_FFT (any other FFT pass).
 enum CvdvKernelKind { KK_FFT = 0, KK_DIAG = 1, KK_QUBIT = 2, KK_CPARITY = 3, KK_REDUCE = 4,
-                      KK_OTHER = 5, KK_COUNT = 6 };
+                      KK_OTHER = 5, KK_FFT_FUSED = 6, KK_FFT_ROW = 7, KK_COUNT = 8 };
 
 struct ProfRec {
     int kind;
     hipEvent_t a, b;
+};
+
+// A register transform left half done (fft.hip).  The four-step FFT of a
+// register field of 2n bits (digits lo = bits [sh, sh+n), hi = [sh+n, sh+2n))
+// is two in-place half passes: H (transform over hi) and L (twiddle W_N^{lo*hi},
+// transform over lo).  After H the second half is pending (!high: L, whose
+// in-place result is the momentum register with its digits exchanged); after L
+// of a second transform, the twiddle and H of that one (high).  The next
+// transform of the same field fuses the pending half with its own first half
+// in one pass (pending phases applied between them); anything else that needs
+// the field in natural order completes it first.
+struct FtDefer {
+    int on = 0;
+    int sh = 0, nb = 0;   // the register field
+    int high = 0;         // pending half: 0 = L (+ digit exchange), 1 = twiddle + H
+    int inv = 0;          // the pending transform is an inverse
+    double scale = 1.0;   // its normalisation, not yet applied
+};
+
+// Every register field with a half-done transform (at most one record per field).
+struct FtDefers {
+    FtDefer f[CVDV_MAX_REG];
+    FtDefer* find(int sh) {
+        for (FtDefer& d : f)
+            if (d.on && d.sh == sh) return &d;
+        return nullptr;
+    }
+    FtDefer* free_slot() {
+        for (FtDefer& d : f)
+            if (!d.on) return &d;
+        return nullptr;
+    }
+    bool any() const {
+        for (const FtDefer& d : f)
+            if (d.on) return true;
+        return false;
+    }
 };
 
 // Host-side state at the start of a graph capture (restored at its end: nothing
@@ -195,6 +235,9 @@ struct CVDVContext {
     bool initialized = false;
     hipStream_t stream = nullptr;
     DiagProg pending{};
+    FtDefers defer{};                 // half-done register transforms (fft.hip)
+    int fuseFt = 1;                   // 0: every transform runs whole (no deferred halves)
+    size_t freeMemOverride = 0;       // readout route choice: assumed free device bytes (0 = ask)
     TwiddleCache* tw = nullptr;
     void* scratch = nullptr;          // readout scratch (grown on demand)
     size_t scratchBytes = 0;
@@ -245,6 +288,15 @@ size_t device_max_lds();
 void fft_generic_c128(CVDVContext* ctx, double2* data, double2* tmp, uint64_t total, int shift,
                       int nbits, bool inverse, double outScale);
 void twiddle_cache_free(TwiddleCache* tc);
+// Complete a half-done transform (no-op when none is pending).  mayAllocate:
+// completing an L half writes the other state buffer; when it does not exist
+// yet and allocation is not allowed, throws CVDV_ERR_STATE.
+void fft_settle(CVDVContext* ctx, bool mayAllocate = true);
+// Complete the half-done transforms whose fields the program reads, outside the
+// bits `except`.
+void fft_settle_for(CVDVContext* ctx, const DiagProg& p, uint64_t except = 0);
+// Complete a half-done transform when the bits `mask` overlap its field.
+void fft_settle_bits(CVDVContext* ctx, uint64_t mask);
 
 // measure.hip
 void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut);

@@ -118,5 +118,9 @@ __device__ __forceinline__ C apply_fast(const FastDiag& f, uint64_t g, C v) {
 FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale = 1.0,
                       bool* scaleFolded = nullptr, uint64_t tileBits = 0);
 
+// compile_diag(p, ..., tileBits) launches nothing but the table build: no
+// FP64 pass and no table pass of its own (with a table slot to spare).
+bool diag_fits(const DiagProg& p, uint64_t tileBits);
+
 // Largest table arena compile_diag can need for registers of at most maxNb qubits.
 size_t diag_arena_bound(int maxNb, size_t elemBytes);

@@ -287,8 +287,11 @@ struct FftPass {
     int vecOut;                    // 1: pairs of complex64 stored as one 16-byte access
     int directOut;                 // 1: last stage stores straight to global (coalesced rows)
     int tw4, tw4Shift, tw4HiBits, tw4LogN;
+    int tw4Uni;                    // 1: the twiddle's second index is the same over the tile
     int usePre, useScale;
     int conjIn, conjOut;           // inverse = conj . forward . conj, split over the passes
+    int conjLoad;                  // conjugate as loaded, before the four-step twiddle (the
+                                   // second half of an inverse whose first half ran alone)
     int lgInH1, lgInH2, lgInR;     // every stride is a power of two
     int lgOutH1, lgOutH2, lgOutR;
     double outScale;  // applied as exact v*hi + rounded v*lo (one FMA): an FP32 store stays unbiased
@@ -546,8 +549,8 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
     if (TW4 && d.tw4) {
         // second four-step pass: straight-line, so these loads issue with the
         // tile's input loads and share their wait
-        if (d.lgHb == 0) {
-            // one row block per tile: k2 is the tile's, so W^{k2 (q + i M/RP)} =
+        if (d.tw4Uni) {
+            // k2 is the tile's (no tile index bit selects it), so W^{k2 (q + i M/RP)} =
             // W^{k2 q} W^{k2 i M/RP} takes one gathered and RP-1 tile-uniform
             // (scalar) loads per butterfly instead of RP gathers
             const uint32_t nmask = (1u << d.tw4LogN) - 1u;
@@ -564,7 +567,8 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(b, st[i]));
             }
         } else {
-            // k2 differs between row blocks: per butterfly, gather W^{k2 q} and
+            // k2 differs within the tile (between row blocks or columns): per
+            // butterfly, gather W^{k2 q} and
             // w = W^{k2 M/RP}; the powers w^i come from a multiplication tree of
             // depth log2(RP)
 #pragma unroll
@@ -935,6 +939,10 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         tile_base(t, lo0, inBase, outBase);
         C x[E];
         stage0_load<R0, E, LGM, LGC>(in, x, d, inBase, tid, nt);
+        if (d.conjLoad) {
+#pragma unroll
+            for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
+        }
         prephase(x, t, tid);
         stages(x, tid, twl);
         store(x, t, tid);
@@ -964,6 +972,8 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             C x[E];
 #pragma unroll
             for (int i = 0; i < E; ++i) x[i] = xn[i];
+            // (no conjLoad here: it belongs to four-step halves, and the
+            // one-row tiles sit at 128 VGPRs already)
             // the phase-table loads wait on the memory counter, which completes in
             // order: issue the next tile's loads after them
             prephase(x, t, ti);
@@ -979,6 +989,92 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             t = tn;
         }
     }
+}
+
+// Two half transforms of one register field in one in-place pass (FtDefer):
+// the pending half of the previous transform (the four-step twiddle on load,
+// a length-M transform over the tile's rows), the pending phases, then the
+// first half of the next transform (a transform over the same rows), stored
+// back in place.  Each half of an inverse is conj . forward half . conj (so a
+// half-done inverse is a linear map of the state and commutes with work on
+// other bits): conjMid conjugates before and after the pending half, dD.conjIn
+// and d.conjOut before and after the next one.
+// The phases are evaluated at the natural index of the completed transform:
+// dD is the tile geometry seen in that index (row and row-block shifts), and
+// swapN > 0 exchanges the two swapN-bit digits at swapSh of the tile base (the
+// in-place L half leaves the register's digits exchanged).  Between the halves
+// the elements go through LDS from the last stage's order to stage 0's.
+template <typename R, int LGM, int LGC, int E>
+__global__ void __launch_bounds__((tile_threads_max<R, LGM, LGC>()))
+k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int conjMid, int swapSh, int swapN,
+            const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
+    using C = typename CT<R>::type;
+    constexpr int NST = Plan<LGM>::NST;
+    constexpr int R0 = 1 << Plan<LGM>::lgr(0);
+    constexpr int RL = 1 << Plan<LGM>::lgr(NST - 1);
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ PreLds P;
+    C* lds = reinterpret_cast<C*>(smem_raw);
+    const int tid = threadIdx.x, nt = blockDim.x;
+
+    if (dD.usePre) {
+        if (nt >= CVDV_FAST_TABS + CVDV_FAST_GEN) {
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_TABS; ++i)
+                if (tid == i && i < pre.ntab) P.tab[i] = pre.tab[i];
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_GEN; ++i)
+                if (tid == CVDV_FAST_TABS + i && i < pre.ngen) P.gen[i] = pre.gen[i];
+        } else if (tid == 0) {
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_TABS; ++i)
+                if (i < pre.ntab) P.tab[i] = pre.tab[i];
+#pragma unroll
+            for (int i = 0; i < CVDV_FAST_GEN; ++i)
+                if (i < pre.ngen) P.gen[i] = pre.gen[i];
+        }
+        __syncthreads();
+    }
+    const R scale = (R)d.outScale;
+    const R scaleLo = (R)(d.outScale - (double)scale);
+
+    const uint64_t t = blockIdx.x;
+    const uint64_t loBlk = t & ((1ull << d.lgNLoBlk) - 1);
+    const uint64_t rr = t >> d.lgNLoBlk;
+    const uint64_t h2Blk = rr & ((1ull << d.lgNH2Blk) - 1);
+    const uint64_t h1 = rr >> d.lgNH2Blk;
+    const uint64_t base = (h1 << d.lgInH1) + ((h2Blk << d.lgHb) << d.lgInH2) + (loBlk << LGC);
+    uint64_t natBase = base;
+    if (swapN > 0) {
+        const uint64_t m = (1ull << swapN) - 1ull;
+        const uint64_t lo = (base >> swapSh) & m, hi = (base >> (swapSh + swapN)) & m;
+        natBase = (base & ~(((m << swapN) | m) << swapSh)) | (((lo << swapN) | hi) << swapSh);
+    }
+
+    C x[E];
+    stage0_load<R0, E, LGM, LGC>(data, x, d, base, tid, nt);
+    if (conjMid) {  // an inverse's half is conj . forward half . conj
+#pragma unroll
+        for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
+    }
+    // pending half: the four-step twiddle (d carries no phases and no input conjugation)
+    stage0_pre<R0, E, LGM, LGC, true, false>(x, d, base, P, pre, tw4, tid, nt, lds);
+    stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
+    run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
+    if (conjMid) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
+    }
+    // last stage's order -> stage 0's order
+    stage_write<RL, E, LGM, LGC, NST - 1>(lds, x, tid, nt);
+    lds_barrier();
+    stage_read<R0, E, LGM, LGC>(lds, x, tid, nt);
+    lds_barrier();
+    // phases at the natural index, then the next transform's first half
+    stage0_pre<R0, E, LGM, LGC, false, true>(x, dD, natBase, P, pre, tw4, tid, nt, lds);
+    stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
+    run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
+    last_store<RL, E, LGM, LGC>(data, x, d, base, scale, scaleLo, tid, nt);
 }
 
 // Direct DFT for very small states (fewer than 8 elements per tile).
@@ -1130,10 +1226,10 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
         // and keep their (split) twiddles in LDS
         // (the second pass of a four-step split, which carries the four-step
-        // twiddle, and a program with a table not read by row run one tile per
-        // workgroup)
+        // twiddle, a pass that conjugates on load, and a program with a table not
+        // read by row run one tile per workgroup)
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
-        const bool pf = PF && !d.tw4 && (!d.usePre || table_by_row(pre, d.lgInR, LGM));
+        const bool pf = PF && !d.tw4 && !d.conjLoad && (!d.usePre || table_by_row(pre, d.lgInR, LGM));
         if (pf) {
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
@@ -1180,7 +1276,7 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
     FftPass dd = d;
     dd.nTiles = blocks;
     const size_t lds = lds_bytes(tile, sizeof(C));
-    const int pid = prof_begin(ctx, KK_FFT);
+    const int pid = prof_begin(ctx, (d.lgC == 0 && d.lgHb == 0 && d.lgM >= 12) ? KK_FFT_ROW : KK_FFT);
     switch (d.lgM) {
 #define CVDV_LGM_CASE(L) \
     case L: launch_lgm<R, L>(ctx, in, out, dd, pre, tw, tw4, blocks, tile, lds); break;
@@ -1192,6 +1288,52 @@ static void launch_pass(CVDVContext* ctx, const typename CT<R>::type* in, typena
     }
     HIPCHK(hipGetLastError());
     prof_end(ctx, pid);
+}
+
+// The four-step split of field [sh, sh+nb) (j = j1 + N1 j2, N1 = 2^(nb/2)):
+//   d1: transform over j2 (stride N1 S), columns (j1, lo), in place;
+//   d2: twiddle W_N^{j1 k2} on load, transform over j1 (stride S), row blocks
+//       k2, written transposed (k2 + N2 k1) out of place;
+// and, for equal digits, the in-place halves a pending transform completes with:
+//   dL:  d2 stored in place (momentum left with its digits exchanged);
+//   d1c: d1 with the twiddle W_N^{j1 j2} (j1 now the tile's) on load.
+struct FourStep {
+    FftPass d1, d2, dL, d1c;
+    uint64_t H;
+    int n1, n2;
+    bool hasL;
+};
+
+static bool four_step_geom(int prec, uint64_t total, int sh, int nb, FourStep& f) {
+    const uint64_t N = 1ull << nb, S = 1ull << sh;
+    const uint64_t H = total / (N * S);
+    if (H == 0 || (H & (H - 1)) != 0 || H * N * S != total) return false;
+    const int n1 = nb / 2, n2 = nb - n1;
+    const uint64_t N1 = 1ull << n1, N2 = 1ull << n2;
+    f = FourStep{};
+    f.H = H;
+    f.n1 = n1;
+    f.n2 = n2;
+    auto uni = [](const FftPass& d, int s, int n) {
+        const uint64_t m = n > 0 ? (((1ull << n) - 1ull) << s) : 0ull;
+        return (tile_bits(d) & m) ? 0 : 1;
+    };
+    PassGeom p1{(int)N2, N1 * S, 1, H, N * S, 0, N1 * S, N * S, 0, N1 * S};
+    PassGeom p2{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, S, N2 * S};
+    if (!fit_pass(p1, prec, f.d1, false) || !fit_pass(p2, prec, f.d2, false)) return false;
+    f.d1.tw4 = 0; f.d1.tw4Shift = 0; f.d1.tw4HiBits = 0; f.d1.tw4LogN = 0; f.d1.tw4Uni = 1;
+    f.d2.tw4 = 1; f.d2.tw4Shift = sh + n1; f.d2.tw4HiBits = n2; f.d2.tw4LogN = nb;
+    f.d2.tw4Uni = uni(f.d2, sh + n1, n2);
+    f.d1c = f.d1;
+    f.d1c.tw4 = 1; f.d1c.tw4Shift = sh; f.d1c.tw4HiBits = n1; f.d1c.tw4LogN = nb;
+    f.d1c.tw4Uni = uni(f.d1c, sh, n1);
+    if (n1 == n2) {
+        PassGeom pL{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, N1 * S, S};
+        f.hasL = fit_pass(pL, prec, f.dL, false);
+        f.dL.tw4 = 1; f.dL.tw4Shift = sh + n1; f.dL.tw4HiBits = n2; f.dL.tw4LogN = nb;
+        f.dL.tw4Uni = uni(f.dL, sh + n1, n2);
+    }
+    return true;
 }
 
 // Runs the FFT over field [sh, sh+nb) of a buffer of `total` elements.
@@ -1248,37 +1390,214 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
         prof_end(ctx, pid);
         return true;
     }
-    // four-step: j = j1 + N1 j2
-    const int n1 = nb / 2, n2 = nb - n1;
-    const uint64_t N1 = 1ull << n1, N2 = 1ull << n2;
-    // pass 1: FFT over j2 (stride N1*S), columns (j1, lo) contiguous, in place
-    PassGeom p1{(int)N2, N1 * S, 1, H, N * S, 0, N1 * S, N * S, 0, N1 * S};
-    FftPass d1{};
-    // pass 2: FFT over j1 (stride S), h2 = k2 (in stride N1*S, out stride S), out rows stride N2*S
-    PassGeom p2{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, S, N2 * S};
-    FftPass d2{};
-    if (!fit_pass(p1, prec, d1, false) || !fit_pass(p2, prec, d2, false))
+    FourStep f;
+    if (!four_step_geom(prec, total, sh, nb, f))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
+    FftPass d1 = f.d1, d2 = f.d2;
     const FastDiag pre = compile_pre(tile_bits(d1));
     const int usePre = act ? 1 : 0;
     // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its
     // output; the twiddle T = W_N^{j1 k2} is applied by pass 2 on load, where the
     // pass has memory time to spare (pass 1 also applies the pending phases)
-    d1.tw4 = 0; d1.tw4Shift = 0; d1.tw4HiBits = 0; d1.tw4LogN = 0;
     d1.usePre = usePre;
     d1.useScale = 0;
     d1.conjIn = inv;
     d1.conjOut = 0;
     d1.outScale = 1.0;
-    d2.tw4 = 1; d2.tw4Shift = sh + n1; d2.tw4HiBits = n2; d2.tw4LogN = nb;
     d2.usePre = 0;
     d2.useScale = folded ? 0 : 1;
     d2.conjIn = 0;
     d2.conjOut = inv;
     d2.outScale = outScale;
     C* tmp = tmpFn();
-    launch_pass<R>(ctx, data, data, d1, pre, prec, H);
-    launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, prec, H);
+    launch_pass<R>(ctx, data, data, d1, pre, prec, f.H);
+    launch_pass<R>(ctx, data, tmp, d2, FastDiag{}, prec, f.H);
+    return true;
+}
+
+// ---------------------------------------------------------------- half transforms kept pending
+// Fused instances exist for digits of 2^FUSE_LGM_MIN .. 2^FUSE_LGM_MAX points
+// (registers of 2^10 .. 2^16 points).
+#define FUSE_LGM_MIN 5
+#define FUSE_LGM_MAX 8
+
+template <typename R, int LGM, int LGC>
+static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, const FftPass& d, const FftPass& dD,
+                              const FastDiag& pre, int conjMid, int swapSh, int swapN,
+                              const typename CT<R>::type* tw, const typename CT<R>::type* tw4, uint64_t blocks,
+                              int tile, size_t lds) {
+    if constexpr (tile_instance<R, LGM, LGC>()) {
+        if (tile / 8 > tile_threads_max<R, LGM, LGC>())
+            throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile larger than its kernel's workgroup bound"};
+        hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8>), dim3((unsigned)blocks), dim3(tile / 8), lds, ctx->stream,
+                           data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
+    } else {
+        throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile shape without a kernel instance"};
+    }
+}
+
+template <typename R, int LGM>
+static void launch_fused_lgm(CVDVContext* ctx, typename CT<R>::type* data, const FftPass& d, const FftPass& dD,
+                             const FastDiag& pre, int conjMid, int swapSh, int swapN,
+                             const typename CT<R>::type* tw, const typename CT<R>::type* tw4, uint64_t blocks,
+                             int tile, size_t lds) {
+    switch (d.lgC) {
+#define CVDV_FUSE_C(L) \
+    case L: launch_fused_geom<R, LGM, L>(ctx, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4, blocks, tile, lds); break;
+        CVDV_FUSE_C(0) CVDV_FUSE_C(1) CVDV_FUSE_C(2) CVDV_FUSE_C(3) CVDV_FUSE_C(4) CVDV_FUSE_C(5)
+#undef CVDV_FUSE_C
+        default: throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile column count out of range"};
+    }
+}
+
+template <typename R>
+static void launch_fused(CVDVContext* ctx, const FftPass& d, const FftPass& dD, const FastDiag& pre, int conjMid,
+                         int swapSh, int swapN, uint64_t nH1) {
+    using C = typename CT<R>::type;
+    const int prec = ctx->prec;
+    const C* tw = (const C*)stage_table(ctx, d.lgM, prec);
+    const C* tw4 = (const C*)full_table(ctx, d.tw4LogN, prec);
+    const int tile = 1 << (d.lgM + d.lgC + d.lgHb);
+    const uint64_t blocks = nH1 << (d.lgNH2Blk + d.lgNLoBlk);
+    FftPass dd = d;
+    dd.nTiles = blocks;
+    const size_t lds = lds_bytes(tile, sizeof(C));
+    C* data = (C*)ctx->state();
+    const int pid = prof_begin(ctx, KK_FFT_FUSED);
+    switch (d.lgM) {
+#define CVDV_FUSE_M(L) \
+    case L: launch_fused_lgm<R, L>(ctx, data, dd, dD, pre, conjMid, swapSh, swapN, tw, tw4, blocks, tile, lds); break;
+        CVDV_FUSE_M(5) CVDV_FUSE_M(6) CVDV_FUSE_M(7) CVDV_FUSE_M(8)
+#undef CVDV_FUSE_M
+        default: throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT digit length out of range"};
+    }
+    HIPCHK(hipGetLastError());
+    prof_end(ctx, pid);
+}
+
+static uint64_t bit_field(int sh, int n) { return n > 0 ? (((1ull << n) - 1ull) << sh) : 0ull; }
+
+// The four-step of this register can leave halves pending: equal digits, fused
+// instances for the digit length, and tiles that store straight to global memory.
+static bool fusable(const CVDVContext* ctx, const FourStep& f, int nb) {
+    return ctx->fuseFt && !ctx->rec && f.n1 == f.n2 && f.n1 >= FUSE_LGM_MIN && f.n1 <= FUSE_LGM_MAX && f.hasL &&
+           f.d1.directOut && f.dL.directOut && nb == 2 * f.n1;
+}
+
+// Complete the half-done transform D (its record already cleared).
+template <typename R>
+static void settle_impl(CVDVContext* ctx, const FtDefer D) {
+    using C = typename CT<R>::type;
+    FourStep f;
+    if (!four_step_geom(ctx->prec, ctx->totalSize, D.sh, D.nb, f))
+        throw CvdvError{CVDV_ERR_STATE, "half-done register transform without its four-step geometry"};
+    C* data = (C*)ctx->state();
+    if (D.high) {
+        // twiddle W_N^{lo hi} and the transform over hi, in place: natural order
+        FftPass d = f.d1c;
+        d.usePre = 0;
+        d.conjLoad = D.inv;
+        d.conjIn = 0;
+        d.conjOut = D.inv;
+        d.useScale = D.scale != 1.0 ? 1 : 0;
+        d.outScale = D.scale;
+        launch_pass<R>(ctx, data, data, d, FastDiag{}, ctx->prec, f.H);
+    } else {
+        // twiddle and the transform over lo, stored transposed into the other buffer
+        FftPass d = f.d2;
+        d.usePre = 0;
+        d.conjLoad = D.inv;
+        d.conjIn = 0;
+        d.conjOut = D.inv;
+        d.useScale = D.scale != 1.0 ? 1 : 0;
+        d.outScale = D.scale;
+        launch_pass<R>(ctx, data, (C*)ctx->other(), d, FastDiag{}, ctx->prec, f.H);
+        ctx->cur = 1 - ctx->cur;
+    }
+}
+
+static void settle_one(CVDVContext* ctx, FtDefer& rec) {
+    const FtDefer D = rec;
+    rec = FtDefer{};
+    if (ctx->prec == CVDV_C128) settle_impl<double>(ctx, D);
+    else settle_impl<float>(ctx, D);
+}
+
+void fft_settle(CVDVContext* ctx, bool mayAllocate) {
+    if (!ctx->defer.any()) return;
+    if (!mayAllocate && !ctx->buf[1 - ctx->cur])
+        for (const FtDefer& d : ctx->defer.f)
+            if (d.on && !d.high)
+                throw CvdvError{CVDV_ERR_STATE,
+                                "a register transform is half done and completing it needs the second state "
+                                "buffer: call cvdvFlush first"};
+    for (FtDefer& d : ctx->defer.f)
+        if (d.on) settle_one(ctx, d);
+}
+
+void fft_settle_bits(CVDVContext* ctx, uint64_t mask) {
+    for (FtDefer& d : ctx->defer.f)
+        if (d.on && (mask & bit_field(d.sh, d.nb))) settle_one(ctx, d);
+}
+
+// A transform of field [sh, sh+nb) that can keep a half pending.  Returns false
+// when the ordinary path must run (the caller has settled nothing yet).
+template <typename R>
+static bool fft_deferred(CVDVContext* ctx, int sh, int nb, bool inverse, const DiagProg& preProg, double outScale) {
+    // only where the ordinary path takes the four-step split (fft_run)
+    const uint64_t N = 1ull << nb, S = 1ull << sh, total = ctx->totalSize;
+    if (total < 4096 || N <= 64 || (total % (N * S)) != 0) return false;
+    {
+        PassGeom g1{(int)N, S, total / (N * S), 1, 0, N * S, S, 0, N * S, S};
+        FftPass d{};
+        if (fit_pass(g1, ctx->prec, d, true)) return false;
+    }
+    FourStep f;
+    if (!four_step_geom(ctx->prec, total, sh, nb, f) || !fusable(ctx, f, nb)) return false;
+    const int n = f.n1;
+    bool act = false, folded = false;
+    if (FtDefer* rec = ctx->defer.find(sh)) {
+        FtDefer& D = *rec;
+        // fuse the pending half with this transform's first half
+        const bool high = D.high != 0;
+        FftPass d = high ? f.d1c : f.dL;
+        FftPass dD = d;  // the tile in the natural index of the completed transform
+        if (!high) {
+            dD.lgInR = sh + n;   // rows: the lo digit here is the momentum's hi digit
+            dD.lgInH2 = sh;      // row blocks: the hi digit here is the momentum's lo digit
+        }
+        const uint64_t natBits = tile_bits(dD);
+        const double scale = outScale * D.scale;
+        if (diag_fits(preProg, natBits)) {
+            const FastDiag pre = compile_diag(ctx, preProg, act, scale, &folded, natBits);
+            d.usePre = 0;
+            d.conjLoad = 0;  // the kernel's conjMid conjugates around the pending half
+            d.conjIn = 0;
+            d.conjOut = inverse ? 1 : 0;
+            d.useScale = folded ? 0 : 1;
+            d.outScale = scale;
+            dD.tw4 = 0;
+            dD.usePre = act ? 1 : 0;
+            dD.conjIn = inverse ? 1 : 0;
+            launch_fused<R>(ctx, d, dD, pre, D.inv, sh, high ? 0 : n, f.H);
+            D = FtDefer{1, sh, nb, high ? 0 : 1, inverse ? 1 : 0, 1.0};
+            return true;
+        }
+        settle_one(ctx, D);
+    }
+    // first half from the natural order: pending phases on load, transform over hi
+    FftPass d = f.d1;
+    const FastDiag pre = compile_diag(ctx, preProg, act, outScale, &folded, tile_bits(d));
+    d.usePre = act ? 1 : 0;
+    d.conjIn = inverse ? 1 : 0;
+    d.conjOut = inverse ? 1 : 0;  // a half-done inverse stays a linear map of the state
+    d.useScale = folded ? 0 : 1;
+    d.outScale = outScale;
+    FtDefer* slot = ctx->defer.free_slot();
+    if (!slot) throw CvdvError{CVDV_ERR_STATE, "no free record for a half-done register transform"};
+    launch_pass<R>(ctx, (typename CT<R>::type*)ctx->state(), (typename CT<R>::type*)ctx->state(), d, pre, ctx->prec,
+                   f.H);
+    *slot = FtDefer{1, sh, nb, 0, inverse ? 1 : 0, 1.0};
     return true;
 }
 
@@ -1297,6 +1616,10 @@ void fft_prepare(CVDVContext* ctx) {
 
 void fft_register(CVDVContext* ctx, int pshift, int nbits, bool inverse, const DiagProg& pre,
                   double outScale) {
+    const bool done = ctx->prec == CVDV_C128 ? fft_deferred<double>(ctx, pshift, nbits, inverse, pre, outScale)
+                                             : fft_deferred<float>(ctx, pshift, nbits, inverse, pre, outScale);
+    if (done) return;
+    if (FtDefer* rec = ctx->defer.find(pshift)) settle_one(ctx, *rec);
     bool swapped;
     if (ctx->prec == CVDV_C128)
         swapped = fft_run<double>(ctx, (double2*)ctx->state(), [ctx] { return (double2*)ctx->other(); },

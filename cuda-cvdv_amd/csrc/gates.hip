@@ -219,6 +219,7 @@ static bool term_is_identity(const DiagTerm& t) {
 // The whole program as one FP64 pass over the state (programs that exceed the
 // compiled capacities).
 static void diag_prog_pass(CVDVContext* ctx, const DiagProg& p) {
+    fft_settle_for(ctx, p);
     const uint64_t n = ctx->totalSize;
     const int pid = prof_begin(ctx, KK_DIAG);
     if (ctx->prec == CVDV_C128)
@@ -356,6 +357,15 @@ static int group_tables(const DiagProg& p, TabGroup* g, uint64_t tileBits) {
     return ng;
 }
 
+bool diag_fits(const DiagProg& p, uint64_t tileBits) {
+    int ngen = 0;
+    for (int i = 0; i < p.n; ++i) ngen += !term_is_identity(p.t[i]) && !term_tabulated(p.t[i]);
+    if (ngen > CVDV_FAST_GEN) return false;
+    TabGroup grp[CVDV_FAST_TABS + 1];
+    // one table slot stays free for the constant phase and scale
+    return group_tables(p, grp, tileBits) < CVDV_FAST_TABS;
+}
+
 size_t diag_arena_bound(int maxNb, size_t elemBytes) {
     const int plain = std::min(maxNb, 20);
     const int merged = std::min(maxNb + CVDV_TAB_SEL, CVDV_TAB_MERGE_BITS);
@@ -487,6 +497,14 @@ static uint64_t term_bits(const DiagTerm& t) {
     return m;
 }
 
+void fft_settle_for(CVDVContext* ctx, const DiagProg& p, uint64_t except) {
+    if (!ctx->defer.any()) return;
+    uint64_t m = 0;
+    for (int i = 0; i < p.n; ++i)
+        if (!term_is_identity(p.t[i])) m |= term_bits(p.t[i]);
+    fft_settle_bits(ctx, m & ~except);
+}
+
 // The pending terms that do not commute with an operation mixing the index
 // bits in `mix` (an FFT along a field, a 2x2 gate on a bit, a conditional
 // reflection of a field).  A diagonal term that depends on none of those bits
@@ -534,6 +552,7 @@ void launch_diag(CVDVContext* ctx, const DiagProg& pp) {
         rec_op(ctx, op, pp);
         return;
     }
+    fft_settle_for(ctx, pp);
     const uint64_t n = ctx->totalSize;
     bool act;
     const FastDiag p = compile_diag(ctx, pp, act);
@@ -549,6 +568,7 @@ void launch_diag(CVDVContext* ctx, const DiagProg& pp) {
 }
 
 void ctx_flush(CVDVContext* ctx) {
+    if (!ctx->rec) fft_settle(ctx);
     if (!prog_active(ctx->pending)) return;
     DiagProg p = ctx_take_pending(ctx);
     launch_diag(ctx, p);
@@ -579,7 +599,11 @@ void ctx_push_term(CVDVContext* ctx, const DiagTerm& t0) {
     }
     bool found = false;
     for (int i = 0; i < ctx->pending.n; ++i) found |= same_slot(ctx->pending.t[i], t);
-    if (!found && ctx->pending.n == CVDV_MAX_TERMS) ctx_flush(ctx);
+    if (!found && ctx->pending.n == CVDV_MAX_TERMS) {
+        // apply what is pending (settling a half-done transform only if it reads its field)
+        const DiagProg p = ctx_take_pending(ctx);
+        launch_diag(ctx, p);
+    }
     prog_add(ctx->pending, t);
 }
 
@@ -649,6 +673,8 @@ void ctx_ft(CVDVContext* ctx, int reg, bool inverse) {
     tw.ctrl = -1;
     tw.c1 = sgn * (double)(N - 1) / (4.0 * (double)N);
     DiagProg pre = ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[reg]);
+    // other registers' half-done transforms whose fields these phases read
+    if (!ctx->rec) fft_settle_for(ctx, pre, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[reg]);
     if (pre.n == CVDV_MAX_TERMS) {
         bool found = false;
         for (int i = 0; i < pre.n; ++i) found |= same_slot(pre.t[i], tw);
@@ -691,8 +717,11 @@ static void qubit_gate(CVDVContext* ctx, int reg, int qubit, const double m[8]) 
         rec_op(ctx, op, ctx_take_involving(ctx, 1ull << bit));
         return;
     }
+    fft_settle_bits(ctx, 1ull << bit);
+    const DiagProg taken = ctx_take_involving(ctx, 1ull << bit);
+    fft_settle_for(ctx, taken);
     bool act;
-    const FastDiag p = compile_diag(ctx, ctx_take_involving(ctx, 1ull << bit), act);
+    const FastDiag p = compile_diag(ctx, taken, act);
     const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const double2 m00 = make_double2(mm[0], mm[1]), m01 = make_double2(mm[2], mm[3]);
@@ -758,9 +787,11 @@ static void cparity(CVDVContext* ctx, int tgt, int creg, int cq) {
         rec_op(ctx, op, ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]));
         return;
     }
+    fft_settle_bits(ctx, ((((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]) | (1ull << cpos));
+    const DiagProg taken = ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]);
+    fft_settle_for(ctx, taken);
     bool act;
-    const FastDiag p =
-        compile_diag(ctx, ctx_take_involving(ctx, (((uint64_t)1 << nb) - 1ull) << ctx->pshift[tgt]), act);
+    const FastDiag p = compile_diag(ctx, taken, act);
     const int usep = act ? 1 : 0;
     const uint64_t np = ctx->totalSize / 2;
     const int pid = prof_begin(ctx, KK_CPARITY);

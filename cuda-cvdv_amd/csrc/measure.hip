@@ -159,6 +159,7 @@ static int popc(uint64_t m) { return __builtin_popcountll(m); }
 
 // Marginal over `regs` (logical registers; output column-major, first fastest).
 void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut) {
+    fft_settle(ctx);  // a half-done register transform is completed first
     const int T = ctx->totalQbt;
     const int TB = T < 8 ? T : 8;
     const uint64_t all = (T >= 64) ? ~0ull : ((1ull << T) - 1ull);
@@ -372,10 +373,6 @@ void cvdvMeasureMultiple(CVDVContext* ctx, const int* regIdxs, int numRegs, doub
     CVDV_TRY marginal_probs(ctx, regIdxs, numRegs, probsOut); CVDV_CATCH()
 }
 
-void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut) {
-    cvdvMeasureMultiple(ctx, regIdxs, numRegs, probsOut);
-}
-
 double cvdvGetNorm(CVDVContext* ctx) {
     // measure.cu:276-300: sum |psi|^2 (not its square root)
     if (!ctx || refuse_recording(ctx)) return 0.0;
@@ -413,6 +410,7 @@ void cvdvGetFidelity(CVDVContext* ctx, void** devicePtrs, int numReg, double* fi
         sd.refl[r] = ctx->refl[r];
     }
     sd.nreg = numReg;
+    fft_settle(ctx);
     HIPCHK(hipDeviceSynchronize());  // register arrays come from the caller's stream
     const uint64_t n = ctx->totalSize;
     const unsigned nb = dot_grid(n);
@@ -472,6 +470,8 @@ void cvdvFidelityStatevectors(CVDVContext* ctx1, CVDVContext* ctx2, double* fidO
         return;
     }
     CVDV_TRY
+    fft_settle(ctx1);
+    fft_settle(ctx2);
     bool same = ctx1->numReg == ctx2->numReg;
     for (int r = 0; same && r < ctx1->numReg; ++r)
         same = ctx1->nq[r] == ctx2->nq[r] && ctx1->pshift[r] == ctx2->pshift[r] &&

@@ -1,0 +1,191 @@
+// measure_ct.hip - the library-reduction comparator for marginals.
+//
+// Replaces the reference's cuTENSOR route (src/cuda/measure.cu:125-256):
+// |psi|^2 of the canonical state into an FP64 buffer, then one hipTensor
+// reduction (opReduce = add, FP64 compute) over the unselected registers,
+// output column-major with the first selected register fastest, as
+// cvdvMeasureMultiple.  It exists to compare and time the own deterministic
+// reduction (measure.hip) against a library one; the summation order is the
+// library's, so results agree to FP64 rounding, not bit for bit.
+//
+// Registers that are adjacent in the state and keep their order in the output
+// (or are both reduced) merge into one tensor mode, so the tensor rank is what
+// the selection needs and no more.
+
+#include <hiptensor/hiptensor.h>
+
+#include <string>
+#include <vector>
+
+#include "cvdv_internal.h"
+#include "../../include/cvdv_hip.h"
+
+template <typename C>
+__global__ void k_abs2(double* __restrict__ out, const C* __restrict__ st, uint64_t n) {
+    for (uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += (uint64_t)gridDim.x * blockDim.x) {
+        const C v = st[g];
+        out[g] = (double)v.x * (double)v.x + (double)v.y * (double)v.y;
+    }
+}
+
+namespace {
+
+struct TensorMode {
+    int64_t len, inStride, outStride;  // outStride < 0: reduced
+    int32_t id;
+};
+
+void ht_check(hiptensorStatus_t s, const char* what) {
+    if (s != HIPTENSOR_STATUS_SUCCESS)
+        throw CvdvError{s == HIPTENSOR_STATUS_NOT_SUPPORTED ? CVDV_ERR_UNSUPPORTED : CVDV_ERR_HIP,
+                        std::string("hipTensor: ") + what + " failed (" + hiptensorGetErrorString(s) + ")"};
+}
+
+// Owns the descriptors of one reduction; released on every path.
+struct Reduction {
+    hiptensorHandle_t h = nullptr;
+    hiptensorTensorDescriptor_t in = nullptr, out = nullptr;
+    hiptensorOperationDescriptor_t op = nullptr;
+    hiptensorPlanPreference_t pref = nullptr;
+    hiptensorPlan_t plan = nullptr;
+    ~Reduction() {
+        if (plan) hiptensorDestroyPlan(plan);
+        if (pref) hiptensorDestroyPlanPreference(pref);
+        if (op) hiptensorDestroyOperationDescriptor(op);
+        if (in) hiptensorDestroyTensorDescriptor(in);
+        if (out) hiptensorDestroyTensorDescriptor(out);
+        if (h) hiptensorDestroy(h);
+    }
+};
+
+// Modes of the canonical state for a selection: registers from least to most
+// significant, each with its extent, input stride and output stride; adjacent
+// registers merge when both are reduced or their output strides continue the
+// same way as their input strides.
+std::vector<TensorMode> tensor_modes(const CVDVContext* ctx, const int* regs, int nsel, uint64_t& outSize) {
+    std::vector<int64_t> ostride(ctx->numReg, -1);
+    outSize = 1;
+    for (int i = 0; i < nsel; ++i) {
+        ostride[regs[i]] = (int64_t)outSize;
+        outSize <<= ctx->nq[regs[i]];
+    }
+    std::vector<TensorMode> modes;
+    for (int r = ctx->numReg - 1; r >= 0; --r) {
+        if (ctx->nq[r] == 0) continue;
+        const int64_t len = 1ll << ctx->nq[r], in = 1ll << ctx->flw[r], out = ostride[r];
+        if (!modes.empty()) {
+            TensorMode& m = modes.back();
+            const bool bothReduced = m.outStride < 0 && out < 0;
+            const bool continues = m.outStride >= 0 && out == m.outStride * m.len && in == m.inStride * m.len;
+            if ((bothReduced && in == m.inStride * m.len) || continues) {
+                m.len *= len;
+                continue;
+            }
+        }
+        modes.push_back(TensorMode{len, in, out, (int32_t)modes.size()});
+    }
+    return modes;
+}
+
+}  // namespace
+
+extern "C" {
+
+void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut) {
+    if (!ctx || !regIdxs || numRegs <= 0 || !probsOut || refuse_recording(ctx)) return;
+    if (!ctx->initialized) {
+        cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
+        return;
+    }
+    uint64_t seen = 0;
+    for (int i = 0; i < numRegs; i++) {
+        if (regIdxs[i] < 0 || regIdxs[i] >= ctx->numReg) {
+            cvdv_set_error(CVDV_ERR_ARG, "Invalid register index: " + std::to_string(regIdxs[i]));
+            return;
+        }
+        if (seen & (1ull << regIdxs[i])) {
+            cvdv_set_error(CVDV_ERR_ARG, "Register listed twice: " + std::to_string(regIdxs[i]));
+            return;
+        }
+        seen |= 1ull << regIdxs[i];
+    }
+    try {
+        ctx_materialize(ctx);  // canonical layout, pending phases applied
+        uint64_t outSize = 0;
+        const std::vector<TensorMode> modes = tensor_modes(ctx, regIdxs, numRegs, outSize);
+        std::vector<int64_t> inLen, inStr, outLen, outStr;
+        std::vector<int32_t> inId, outId;
+        // The reduction keeps the selected modes in the state's own order (a
+        // permuted output of four or more modes came back wrong from the library);
+        // the result, at most a few MiB, is then permuted on the host.
+        std::vector<const TensorMode*> kept;
+        int64_t compact = 1;
+        for (const TensorMode& m : modes) {
+            inLen.push_back(m.len);
+            inStr.push_back(m.inStride);
+            inId.push_back(m.id);
+            if (m.outStride >= 0) {
+                kept.push_back(&m);
+                outLen.push_back(m.len);
+                outStr.push_back(compact);
+                outId.push_back(m.id);
+                compact *= m.len;
+            }
+        }
+        const uint64_t n = ctx->totalSize;
+        double* sq = (double*)ctx_scratch(ctx, (n + outSize) * sizeof(double));
+        double* res = sq + n;
+        Reduction R;
+        ht_check(hiptensorCreate(&R.h), "hiptensorCreate");
+        ht_check(hiptensorCreateTensorDescriptor(R.h, &R.in, (uint32_t)inLen.size(), inLen.data(), inStr.data(),
+                                                 HIPTENSOR_R_64F, 256),
+                 "input descriptor");
+        ht_check(hiptensorCreateTensorDescriptor(R.h, &R.out, (uint32_t)outLen.size(), outLen.data(),
+                                                 outStr.data(), HIPTENSOR_R_64F, 256),
+                 "output descriptor");
+        ht_check(hiptensorCreateReduction(R.h, &R.op, R.in, inId.data(), HIPTENSOR_OP_IDENTITY, R.out, outId.data(),
+                                          HIPTENSOR_OP_IDENTITY, R.out, outId.data(), HIPTENSOR_OP_ADD,
+                                          HIPTENSOR_COMPUTE_DESC_64F),
+                 "reduction descriptor");
+        ht_check(hiptensorCreatePlanPreference(R.h, &R.pref, HIPTENSOR_ALGO_DEFAULT, HIPTENSOR_JIT_MODE_NONE),
+                 "plan preference");
+        uint64_t ws = 0;
+        ht_check(hiptensorEstimateWorkspaceSize(R.h, R.op, R.pref, HIPTENSOR_WORKSPACE_DEFAULT, &ws), "workspace size");
+        ht_check(hiptensorCreatePlan(R.h, &R.plan, R.op, R.pref, ws), "plan");
+        void* wsp = nullptr;
+        if (ws > 0) HIPCHK(hipMalloc(&wsp, ws));
+        HIPCHK(hipMemsetAsync(res, 0, outSize * sizeof(double), ctx->stream));  // C of D = a A + b C
+        const int pid = prof_begin(ctx, KK_REDUCE);
+        if (ctx->prec == CVDV_C128)
+            hipLaunchKernelGGL(k_abs2<double2>, dim3(2048), dim3(256), 0, ctx->stream, sq, (const double2*)ctx->state(), n);
+        else
+            hipLaunchKernelGGL(k_abs2<float2>, dim3(2048), dim3(256), 0, ctx->stream, sq, (const float2*)ctx->state(), n);
+        hipError_t le = hipGetLastError();
+        const double one = 1.0, zero = 0.0;
+        hiptensorStatus_t rs = HIPTENSOR_STATUS_SUCCESS;
+        if (le == hipSuccess) rs = hiptensorReduce(R.h, R.plan, &one, sq, &zero, res, res, wsp, ws, ctx->stream);
+        prof_end(ctx, pid);
+        hipError_t se = hipStreamSynchronize(ctx->stream);
+        if (wsp) (void)hipFree(wsp);
+        HIPCHK(le);
+        ht_check(rs, "hiptensorReduce");
+        HIPCHK(se);
+        std::vector<double> natural(outSize);
+        HIPCHK(hipMemcpy(natural.data(), res, outSize * sizeof(double), hipMemcpyDeviceToHost));
+        // natural index (kept modes, least significant first) -> requested column-major index
+        for (uint64_t e = 0; e < outSize; ++e) {
+            uint64_t rem = e, o = 0;
+            for (const TensorMode* m : kept) {
+                o += (rem % (uint64_t)m->len) * (uint64_t)m->outStride;
+                rem /= (uint64_t)m->len;
+            }
+            probsOut[o] = natural[e];
+        }
+    } catch (const CvdvError& e) {
+        cvdv_set_error(e.code, e.msg);
+    } catch (...) {
+        cvdv_set_error(CVDV_ERR_HIP, "unexpected internal error");
+    }
+}
+
+}  // extern "C"

@@ -230,22 +230,30 @@ static void wigner_device(CVDVContext* ctx, int r, void* dev, bool asComplex, do
     HIPCHK(hipGetLastError());
 }
 
-static void get_wigner(CVDVContext* ctx, int r, double* out) {
+// The readouts below write their N x N result into `out`: host memory (through
+// a device scratch copy), or, with outOnDevice, caller device memory directly.
+static void finish_out(CVDVContext* ctx, double* out, const double* dev, uint64_t n, bool outOnDevice) {
+    if (!outOnDevice) HIPCHK(hipMemcpyAsync(out, dev, n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+static void get_wigner(CVDVContext* ctx, int r, double* out, bool outOnDevice) {
     const int N = 1 << ctx->nq[r];
     const uint64_t N2 = (uint64_t)N * N, A = ctx->totalSize;
     // scratch: X (A c128) | rho (N2) | G (N2) | tmp (N2) | W (N2 doubles)
-    char* base = (char*)ctx_scratch(ctx, A * 16 + 3 * N2 * 16 + N2 * 8);
+    char* base = (char*)ctx_scratch(ctx, A * 16 + 3 * N2 * 16 + (outOnDevice ? 0 : N2 * 8));
     double2* X = (double2*)base;
     double2* rho = X + A;
     double2* G = rho + N2;
     double2* tmp = G + N2;
-    double* W = (double*)(tmp + N2);
+    double* W = outOnDevice ? out : (double*)(tmp + N2);
+    const int pid = prof_begin(ctx, KK_OTHER);  // the readout's device time, first to last kernel
     wigner_device(ctx, r, W, false, X, rho, G, tmp);
-    HIPCHK(hipMemcpyAsync(out, W, N2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_end(ctx, pid);
+    finish_out(ctx, out, W, N2, outOnDevice);
 }
 
-static void get_husimi_overlap(CVDVContext* ctx, int r, double* out) {
+static void get_husimi_overlap(CVDVContext* ctx, int r, double* out, bool outOnDevice) {
     ctx_flush(ctx);
     const int n = ctx->nq[r], N = 1 << n;
     const uint64_t N2 = (uint64_t)N * N, A = ctx->totalSize, S = A >> n;
@@ -254,12 +262,13 @@ static void get_husimi_overlap(CVDVContext* ctx, int r, double* out) {
     while (Qc * 2 <= (uint64_t)N && Qc * 2 * S * N <= (1ull << 26)) Qc *= 2;
     const uint64_t batch = Qc * S * N;
     // scratch: X (A) | B (batch) | tmp (batch) | acc (N2 doubles) | Q (N2 doubles)
-    char* base = (char*)ctx_scratch(ctx, A * 16 + 2 * batch * 16 + 2 * N2 * 8);
+    char* base = (char*)ctx_scratch(ctx, A * 16 + 2 * batch * 16 + (outOnDevice ? 1 : 2) * N2 * 8);
     double2* X = (double2*)base;
     double2* B = X + A;
     double2* tmp = B + batch;
     double* acc = (double*)(tmp + batch);
-    double* Q = acc + N2;
+    double* Q = outOnDevice ? out : acc + N2;
+    const int pid = prof_begin(ctx, KK_OTHER);
     Bufs b = extract_slices(ctx, r, X);
     for (uint64_t q0 = 0; q0 < (uint64_t)N; q0 += Qc) {
         hipLaunchKernelGGL(k_husimi_window, dim3(grid_for(batch)), dim3(256), 0, ctx->stream, B, X, b.S, n,
@@ -272,21 +281,22 @@ static void get_husimi_overlap(CVDVContext* ctx, int r, double* out) {
     }
     hipLaunchKernelGGL(k_husimi_finalize, dim3(grid_for(N2)), dim3(256), 0, ctx->stream, Q, acc, N);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, Q, N2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_end(ctx, pid);
+    finish_out(ctx, out, Q, N2, outOnDevice);
 }
 
 // bosonic.cu:63-101: Wigner -> 2D FFT -> Gaussian -> inverse 2D FFT -> Re/N^2, clamped
-static void get_husimi_wigner(CVDVContext* ctx, int r, double* out) {
+static void get_husimi_wigner(CVDVContext* ctx, int r, double* out, bool outOnDevice) {
     const int n = ctx->nq[r], N = 1 << n;
     const uint64_t N2 = (uint64_t)N * N, A = ctx->totalSize;
-    char* base = (char*)ctx_scratch(ctx, A * 16 + 4 * N2 * 16 + N2 * 8);
+    char* base = (char*)ctx_scratch(ctx, A * 16 + 4 * N2 * 16 + (outOnDevice ? 0 : N2 * 8));
     double2* X = (double2*)base;
     double2* rho = X + A;
     double2* G = rho + N2;
     double2* tmp = G + N2;
     double2* Wc = tmp + N2;
-    double* Q = (double*)(Wc + N2);
+    double* Q = outOnDevice ? out : (double*)(Wc + N2);
+    const int pid = prof_begin(ctx, KK_OTHER);
     wigner_device(ctx, r, Wc, true, X, rho, G, tmp);
     if (n > 0) {
         fft_generic_c128(ctx, Wc, tmp, N2, 0, n, false, 1.0);  // along q (columns index)
@@ -300,8 +310,25 @@ static void get_husimi_wigner(CVDVContext* ctx, int r, double* out) {
     }
     hipLaunchKernelGGL(k_real_clamp, dim3(grid_for(N2)), dim3(256), 0, ctx->stream, Q, Wc, N2, (double)N2);
     HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(out, Q, N2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHK(hipStreamSynchronize(ctx->stream));
+    prof_end(ctx, pid);
+    finish_out(ctx, out, Q, N2, outOnDevice);
+}
+
+// bosonic.cu:13-25: the overlap route holds one N-point window, two N x N double
+// arrays, every slice and an N x N complex block per slice; above 80% of the
+// free device memory (or when the device cannot say) the FFT2 route runs.  Both
+// are FP64; they differ in grid and normalisation (getHusimiQWigner's p spacing
+// is pi/(N dx) and its scale that of the Wigner), as in the reference.  The
+// overlap route here is chunked and needs far less than that estimate; the
+// estimate is kept so the same states take the same route.
+static void get_husimi(CVDVContext* ctx, int r, double* out, bool outOnDevice) {
+    const uint64_t N = 1ull << ctx->nq[r], N2 = N * N, S = ctx->totalSize >> ctx->nq[r];
+    const double need = (double)(N * 16 + 2 * N2 * 8) + (double)S * (double)N * 16.0 + (double)S * (double)N2 * 16.0;
+    size_t freeB = ctx->freeMemOverride, totalB = 0;
+    bool known = true;
+    if (!freeB) known = hipMemGetInfo(&freeB, &totalB) == hipSuccess;
+    if (!known || need > 0.8 * (double)freeB) get_husimi_wigner(ctx, r, out, outOnDevice);
+    else get_husimi_overlap(ctx, r, out, outOnDevice);
 }
 
 #define CVDV_TRY try {
@@ -331,22 +358,37 @@ extern "C" {
 
 void cvdvGetWigner(CVDVContext* ctx, int regIdx, double* wignerOut) {
     if (!ps_ok(ctx, regIdx, wignerOut)) return;
-    CVDV_TRY get_wigner(ctx, regIdx, wignerOut); CVDV_CATCH()
+    CVDV_TRY get_wigner(ctx, regIdx, wignerOut, false); CVDV_CATCH()
 }
 
 void cvdvGetHusimiQ(CVDVContext* ctx, int regIdx, double* husimiOut) {
     if (!ps_ok(ctx, regIdx, husimiOut)) return;
-    CVDV_TRY get_husimi_overlap(ctx, regIdx, husimiOut); CVDV_CATCH()
+    CVDV_TRY get_husimi(ctx, regIdx, husimiOut, false); CVDV_CATCH()
 }
 
 void cvdvGetHusimiQOverlap(CVDVContext* ctx, int regIdx, double* husimiOut) {
     if (!ps_ok(ctx, regIdx, husimiOut)) return;
-    CVDV_TRY get_husimi_overlap(ctx, regIdx, husimiOut); CVDV_CATCH()
+    CVDV_TRY get_husimi_overlap(ctx, regIdx, husimiOut, false); CVDV_CATCH()
 }
 
 void cvdvGetHusimiQWigner(CVDVContext* ctx, int regIdx, double* husimiOut) {
     if (!ps_ok(ctx, regIdx, husimiOut)) return;
-    CVDV_TRY get_husimi_wigner(ctx, regIdx, husimiOut); CVDV_CATCH()
+    CVDV_TRY get_husimi_wigner(ctx, regIdx, husimiOut, false); CVDV_CATCH()
+}
+
+void cvdvGetWignerDevice(CVDVContext* ctx, int regIdx, double* d_out) {
+    if (!ps_ok(ctx, regIdx, d_out)) return;
+    CVDV_TRY get_wigner(ctx, regIdx, d_out, true); CVDV_CATCH()
+}
+
+void cvdvGetHusimiQDevice(CVDVContext* ctx, int regIdx, double* d_out) {
+    if (!ps_ok(ctx, regIdx, d_out)) return;
+    CVDV_TRY get_husimi(ctx, regIdx, d_out, true); CVDV_CATCH()
+}
+
+void cvdvSetFreeMemoryOverride(CVDVContext* ctx, size_t bytes) {
+    if (!ctx) return;
+    ctx->freeMemOverride = bytes;
 }
 
 }  // extern "C"

@@ -53,6 +53,9 @@ SIGNATURES = {
     "cvdvGetHusimiQ": ([c_void_p, c_int, POINTER(c_double)], None),
     "cvdvGetHusimiQOverlap": ([c_void_p, c_int, POINTER(c_double)], None),
     "cvdvGetHusimiQWigner": ([c_void_p, c_int, POINTER(c_double)], None),
+    "cvdvGetWignerDevice": ([c_void_p, c_int, c_void_p], None),
+    "cvdvGetHusimiQDevice": ([c_void_p, c_int, c_void_p], None),
+    "cvdvSetFreeMemoryOverride": ([c_void_p, c_size_t], None),
     "cvdvMeasureMultiple": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
     "cvdvMeasureMultipleCT": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
     "cvdvGetState": ([c_void_p, POINTER(c_double), POINTER(c_double)], None),
@@ -64,6 +67,7 @@ SIGNATURES = {
     "cvdvGetLastError": ([ctypes.c_char_p, c_size_t], c_int),
     "cvdvSynchronize": ([c_void_p], None),
     "cvdvFlush": ([c_void_p], None),
+    "cvdvSetTransformFusion": ([c_void_p, c_int], None),
     "cvdvGetStateDevice": ([c_void_p, c_void_p], None),
     "cvdvProfileEnable": ([c_void_p, c_int], None),
     "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),

@@ -229,31 +229,43 @@ class HipCvdv:
     def getPhotonNumber(self, regIdx: int) -> float:
         return float(self._c("cvdvGetPhotonNumber", c_int(regIdx)))
 
-    def getWigner(self, regIdx: int, bound: float | None = None) -> npt.NDArray[np.float64]:
-        dim = self.register_dims[regIdx]
-        out = np.zeros(dim * dim, dtype=np.float64)
-        self._c("cvdvGetWigner", regIdx, out.ctypes.data_as(POINTER(c_double)))
-        W = out.reshape((dim, dim))
-        if bound is None:
-            return W
-        dx = self.grid_steps[regIdx]
-        dp = np.pi / (dim * dx)
-        x_vals = (np.arange(dim) - (dim - 1) / 2) * dx
-        p_vals = (np.arange(dim) - dim / 2) * dp
-        return W[np.ix_(np.abs(p_vals) <= bound, np.abs(x_vals) <= bound)]
+    # ------------------------------------------------------------ phase space (cudaCvdv.py:186-210)
+    def _phase_axes(self, regIdx: int, p_factor: float):
+        """Axes of the native N x N readouts: x_j = (j - (N-1)/2) dx and
+        p_k = (k - N/2) p_factor pi / (N dx) (Wigner: p_factor 1; Husimi: 2)."""
+        n = self.register_dims[regIdx]
+        dx = float(self.grid_steps[regIdx])
+        xs = dx * (np.arange(n) - 0.5 * (n - 1))
+        ps = (p_factor * np.pi / (n * dx)) * (np.arange(n) - 0.5 * n)
+        return xs, ps
 
-    def getHusimiQ(self, regIdx: int, bound: float | None = None) -> npt.NDArray[np.float64]:
-        dim = self.register_dims[regIdx]
-        out = np.zeros(dim * dim, dtype=np.float64)
-        self._c("cvdvGetHusimiQ", regIdx, out.ctypes.data_as(POINTER(c_double)))
-        Q = out.reshape((dim, dim))
+    def _native_readout(self, entry: str, regIdx: int, bound, p_factor: float, device: bool):
+        if not 0 <= regIdx < self.num_registers:
+            raise ValueError(f"Invalid register index: {regIdx}")
+        n = self.register_dims[regIdx]
+        if device:
+            out = torch.empty((n, n), dtype=torch.float64, device="cuda")
+            self._c(entry + "Device", c_int(regIdx), _dptr(out))
+        else:
+            out = np.empty((n, n), dtype=np.float64)
+            self._c(entry, c_int(regIdx), out.ctypes.data_as(POINTER(c_double)))
         if bound is None:
-            return Q
-        dx = self.grid_steps[regIdx]
-        dp = 2 * np.pi / (dim * dx)
-        q_vals = (np.arange(dim) - (dim - 1) / 2) * dx
-        p_vals = (np.arange(dim) - dim / 2) * dp
-        return Q[np.ix_(np.abs(p_vals) <= bound, np.abs(q_vals) <= bound)]
+            return out
+        xs, ps = self._phase_axes(regIdx, p_factor)
+        rows, cols = np.flatnonzero(np.abs(ps) <= bound), np.flatnonzero(np.abs(xs) <= bound)
+        if device:
+            return out[torch.from_numpy(rows).to(out.device)][:, torch.from_numpy(cols).to(out.device)]
+        return out[np.ix_(rows, cols)]
+
+    def getWigner(self, regIdx: int, bound: float | None = None, device: bool = False):
+        """Native-grid Wigner of register regIdx traced over the others: (N_p, N_x)
+        rows momentum; `bound` keeps |x|, |p| <= bound.  device=True (an addition)
+        returns a CUDA float64 tensor and skips the copy into host memory."""
+        return self._native_readout("cvdvGetWigner", regIdx, bound, 1.0, device)
+
+    def getHusimiQ(self, regIdx: int, bound: float | None = None, device: bool = False):
+        """Native-grid Husimi-Q (route chosen as cvdvGetHusimiQ does); as getWigner."""
+        return self._native_readout("cvdvGetHusimiQ", regIdx, bound, 2.0, device)
 
     def getHusimiQWigner(self, regIdx: int) -> npt.NDArray[np.float64]:
         """Husimi Q by Gaussian smoothing of the native Wigner grid (bosonic.cu:63-101)."""
@@ -308,45 +320,34 @@ class HipCvdv:
                 "precision": self.precision}
 
     # ------------------------------------------------------------ plots (cudaCvdv.py:212-252)
-    def plotWigner(self, regIdx: int, bound: float = 5.0, cmap: str = "RdBu",
-                   figsize: Tuple[int, int] = (7, 6), show: bool = True) -> Tuple[Any, Any]:
+    def _plot_field(self, field, xs, ps, bound, cmap, figsize, show, signed):
         import matplotlib.pyplot as plt
-        W = self.getWigner(regIdx)
-        dim, dx = self.register_dims[regIdx], self.grid_steps[regIdx]
-        dp = np.pi / (dim * dx)
-        xv = (np.arange(dim) - (dim - 1) / 2) * dx
-        pv = (np.arange(dim) - dim / 2) * dp
-        xm, pm = np.abs(xv) <= bound, np.abs(pv) <= bound
+
+        rows, cols = np.abs(ps) <= bound, np.abs(xs) <= bound
+        part = field[np.ix_(rows, cols)]
+        span = {}
+        if signed:  # a colour scale symmetric about zero
+            top = float(np.max(np.abs(part)))
+            span = {"vmin": -top, "vmax": top}
         fig, ax = plt.subplots(figsize=figsize)
-        Wp = W[np.ix_(pm, xm)]
-        vmax = np.max(np.abs(Wp))
-        im = ax.pcolormesh(xv[xm], pv[pm], Wp, cmap=cmap, vmin=-vmax, vmax=vmax, shading="auto")
+        mesh = ax.pcolormesh(xs[cols], ps[rows], part, cmap=cmap, shading="auto", **span)
         ax.set_xlabel(r"$q$")
         ax.set_ylabel(r"$p$")
-        fig.colorbar(im, ax=ax)
-        plt.tight_layout()
+        fig.colorbar(mesh, ax=ax)
+        fig.tight_layout()
         if show:
             plt.show()
         return fig, ax
 
+    def plotWigner(self, regIdx: int, bound: float = 5.0, cmap: str = "RdBu",
+                   figsize: Tuple[int, int] = (7, 6), show: bool = True) -> Tuple[Any, Any]:
+        xs, ps = self._phase_axes(regIdx, 1.0)
+        return self._plot_field(self.getWigner(regIdx), xs, ps, bound, cmap, figsize, show, signed=True)
+
     def plotHusimiQ(self, regIdx: int, bound: float = 5.0, cmap: str = "viridis",
                     figsize: Tuple[int, int] = (7, 6), show: bool = True) -> Tuple[Any, Any]:
-        import matplotlib.pyplot as plt
-        Q = self.getHusimiQ(regIdx)
-        dim, dx = self.register_dims[regIdx], self.grid_steps[regIdx]
-        dp = 2 * np.pi / (dim * dx)
-        qv = (np.arange(dim) - (dim - 1) / 2) * dx
-        pv = (np.arange(dim) - dim / 2) * dp
-        qm, pm = np.abs(qv) <= bound, np.abs(pv) <= bound
-        fig, ax = plt.subplots(figsize=figsize)
-        im = ax.pcolormesh(qv[qm], pv[pm], Q[np.ix_(pm, qm)], cmap=cmap, shading="auto")
-        ax.set_xlabel(r"$q$")
-        ax.set_ylabel(r"$p$")
-        fig.colorbar(im, ax=ax)
-        plt.tight_layout()
-        if show:
-            plt.show()
-        return fig, ax
+        xs, ps = self._phase_axes(regIdx, 2.0)
+        return self._plot_field(self.getHusimiQ(regIdx), xs, ps, bound, cmap, figsize, show, signed=False)
 
 
 class HipCvdvGraph:

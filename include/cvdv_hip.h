@@ -73,9 +73,12 @@ void cvdvGetHusimiQWigner(CVDVContext* ctx, int regIdx, double* husimiOut);  /* 
 /* ---- measurement and state readout (api.h:53-61) ---- */
 /* probsOut holds prod(dim) doubles, column-major: first selected register fastest. */
 void   cvdvMeasureMultiple(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut); /* api.h:53-54 */
-/* The reference's cuTENSOR comparator (api.h:55-56); here the same deterministic
- * reduction as cvdvMeasureMultiple, kept for ABI compatibility. */
-void   cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut);
+/* The library-reduction comparator (api.h:55-56; the reference's cuTENSOR
+ * route): |psi|^2 of the canonical state reduced by one hipTensor reduction,
+ * same output as cvdvMeasureMultiple to FP64 rounding (the library's summation
+ * order, not deterministic).  For comparisons and timing; selections beyond
+ * what hipTensor supports fail with CVDV_ERR_UNSUPPORTED. */
+void   cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut); /* api.h:55-56 */
 void   cvdvGetState(CVDVContext* ctx, double* realOut, double* imagOut);    /* api.h:57 */
 double cvdvGetNorm(CVDVContext* ctx);                                      /* api.h:58 */
 void   cvdvGetFidelity(CVDVContext* ctx, void** devicePtrs, int numReg, double* fidOut); /* api.h:59 */
@@ -96,15 +99,40 @@ void   cvdvSynchronize(CVDVContext* ctx);
 /* Applies any deferred diagonal phases and restores the canonical register
  * layout (both are otherwise applied lazily by the next kernel). */
 void   cvdvFlush(CVDVContext* ctx);
+/* Consecutive Fourier transforms of one register that the four-step split
+ * serves (registers of 2^10 .. 2^16 points whose tile does not hold them whole)
+ * share HBM passes: each transform runs as two half passes, and the second half
+ * of one is fused with the first half of the next, the phases between them
+ * applied in the same pass (m transforms: m + 1 passes instead of 2m).  on = 0
+ * runs every transform whole (two passes each), the reference's pass structure;
+ * results agree to FFT rounding.  Default on. */
+void   cvdvSetTransformFusion(CVDVContext* ctx, int on);
 /* Copies the state (canonical layout, complex128) to a caller device buffer. */
 void   cvdvGetStateDevice(CVDVContext* ctx, void* d_dst);
 /* Per-launch timing with HIP events recorded on the context's stream.
- * kind: 0 FFT passes, 1 diagonal passes, 2 qubit-pair passes, 3 conditional
- * parity, 4 reductions, 5 layout/other.  Read returns the launch count and the
- * summed (and largest) elapsed milliseconds since the last enable/reset. */
+ * kind: 0 FFT passes (all of them), 1 diagonal passes, 2 qubit-pair passes,
+ * 3 conditional parity, 4 reductions, 5 layout/other, and among the FFT
+ * passes: 6 fused half-transform passes (cvdvSetTransformFusion), 7 one-row
+ * tiles (a whole contiguous register of 2^12 points or more per tile).  Read
+ * returns the launch count and the summed (and largest) elapsed milliseconds
+ * since the last enable/reset. */
 void   cvdvProfileEnable(CVDVContext* ctx, int on);
 int    cvdvProfileRead(CVDVContext* ctx, int kind, double* totalMs, double* maxMs);
 void   cvdvProfileReset(CVDVContext* ctx);
+
+/* ---- phase-space readouts into device memory (addition) ----
+ * cvdvGetWigner / cvdvGetHusimiQ (same arrays, same route choice) written to
+ * caller device memory (N x N doubles); they return when the result is
+ * complete.  The host-output forms spend most of their time copying an N x N
+ * result into pageable host memory (128 MiB at N = 4096). */
+void   cvdvGetWignerDevice(CVDVContext* ctx, int regIdx, double* d_out);
+void   cvdvGetHusimiQDevice(CVDVContext* ctx, int regIdx, double* d_out);
+/* cvdvGetHusimiQ picks its route as the reference does (bosonic.cu:13-25): the
+ * overlap route unless its memory estimate exceeds 80% of the free device
+ * memory, then the Wigner -> 2D FFT route (cvdvGetHusimiQWigner: other grid and
+ * scale).  bytes > 0 replaces the device's free-memory figure in that choice;
+ * 0 asks the device (the default). */
+void   cvdvSetFreeMemoryOverride(CVDVContext* ctx, size_t bytes);
 
 /* ---- graph capture of a gate sequence (addition) ----
  * cvdvGraphBegin brings the context to its canonical layout with no pending

@@ -599,6 +599,27 @@ class CvdvOracle:
         Q = np.abs(H) ** 2 * (dx / PI ** 1.5)
         return Q if per_slice else Q.sum(axis=0)
 
+    def getHusimiQWigner(self, regIdx: int) -> np.ndarray:
+        """The CUDA backend's fallback Husimi route (bosonic.cu:63-101, kernels
+        bosonic.cuh:158-186), which TorchCvdv lacks: the native Wigner (getWigner,
+        rows p with spacing pi/(N dx), columns x), its 2D DFT times
+        g = exp(-(wq^2 + wp^2)/4) with wq = 2 pi kq/(N dx) along x and
+        wp = 2 pi kp/(N dp) along p (signed frequencies), the inverse 2D DFT, its
+        real part clamped at 0.  A circular convolution of W with a Gaussian of
+        variance 1/2 per axis: on the Wigner's grid, not getHusimiQ's, and
+        normalised like W (a coherent state gives dx/(2 pi) e^{-|z - z0|^2/2})."""
+        dim = self.register_dims[regIdx]
+        dx = self.grid_steps[regIdx]
+        W = self.getWigner(regIdx)
+        k = np.fft.fftfreq(dim) * dim  # signed index: k for k <= N/2, k - N above
+        k[dim // 2] = dim // 2          # bosonic.cuh:166-167 keeps +N/2 positive
+        dp = PI / (dim * dx)
+        wq = 2 * PI * k / (dim * dx)
+        wp = 2 * PI * k / (dim * dp)
+        g = np.exp(-0.25 * (wp[:, None] ** 2 + wq[None, :] ** 2))
+        Q = np.fft.ifft2(np.fft.fft2(W) * g).real
+        return np.maximum(Q, 0.0)
+
     def getHusimiQ(self, regIdx: int, bound: float | None = None) -> np.ndarray:
         # torchCvdv.py:376-404 (non-periodic Gaussian window)
         dim = self.register_dims[regIdx]

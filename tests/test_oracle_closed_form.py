@@ -277,6 +277,28 @@ def test_wigner_and_husimi_closed_form():
     assert np.all(Q >= -1e-14)
 
 
+@pytest.mark.parametrize("n,a", [(8, 1.0 + 0.5j), (9, 0.8 - 1.0j), (10, 1.5 + 1.0j)])
+def test_husimi_fft2_route_closed_form(n, a):
+    """getHusimiQWigner (bosonic.cu:63-101): the native Wigner of a coherent state,
+    W = dx/pi e^{-(x-q0)^2-(p-p0)^2} cos(p dx) (the (N-1) finalisation), smoothed by
+    a Gaussian of variance 1/2 per axis.  Along p the cosine rides through the
+    convolution: e^{-(p-p0)^2} cos(p dx) -> e^{-(p-p0)^2/2 - dx^2/8} cos(dx (p+p0)/2) / sqrt2,
+    so Q = dx/(2 pi) e^{-(x-q0)^2/2 - (p-p0)^2/2 - dx^2/8} cos(dx (p + p0)/2), clamped at 0,
+    on the Wigner's grid (p spacing pi/(N dx)).  Swapping the two axes' frequency
+    scales, dropping the clamp or the inverse normalisation fails this."""
+    s, _ = _sim([n], [["setCoherent", 0, a]], conv="cuda")
+    Q = s.getHusimiQWigner(0)
+    N = 1 << n
+    dx = s.grid_steps[0]
+    dp = pi / (N * dx)
+    X, P = np.meshgrid((np.arange(N) - (N - 1) / 2) * dx, (np.arange(N) - N / 2) * dp)
+    q0, p0 = sqrt(2) * a.real, sqrt(2) * a.imag
+    ex = dx / (2 * pi) * np.exp(-0.5 * (X - q0) ** 2 - 0.5 * (P - p0) ** 2 - dx * dx / 8) * np.cos(dx * (P + p0) / 2)
+    ex = np.maximum(ex, 0.0)
+    assert np.max(np.abs(Q - ex)) <= 1e-12 * ex.max()
+    assert np.all(Q >= 0)
+
+
 @pytest.mark.parametrize("alpha", [0.0, 1.0, 1.5 + 1.0j, -0.5 + 2.0j])
 def test_photon_number_coherent(alpha):
     s, _ = _sim([8], [["setCoherent", 0, alpha]])

@@ -16,6 +16,8 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -158,8 +160,14 @@ __global__ void __launch_bounds__(256) k_merge_splits_bin(const double* __restri
 static int popc(uint64_t m) { return __builtin_popcountll(m); }
 
 // Marginal over `regs` (logical registers; output column-major, first fastest).
-void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut) {
-    fft_settle(ctx);  // a half-done register transform is completed first
+// The per-split partials go to `fixedScratch` (capElems doubles; the split count
+// is capped to fit, and a marginal whose bins alone do not fit is refused) or,
+// when it is null, to the context's scratch buffer (grown on demand).
+static void marginal_core(CVDVContext* ctx, const int* regs, int nsel, double* hostOut, double* fixedScratch,
+                          size_t capElems) {
+    // a half-done register transform is completed first (without allocating on
+    // the fixed-scratch path)
+    fft_settle(ctx, fixedScratch == nullptr);
     const int T = ctx->totalQbt;
     const int TB = T < 8 ? T : 8;
     const uint64_t all = (T >= 64) ? ~0ull : ((1ull << T) - 1ull);
@@ -184,17 +192,25 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
     a.outSize = outSize;
     const uint64_t nGroups = 1ull << popc(a.wgSmask);
     const uint64_t nLoop = 1ull << popc(a.loopUmask);
+    if (fixedScratch && outSize > capElems)
+        throw CvdvError{CVDV_ERR_ARG, "marginal of " + std::to_string(outSize) + " bins exceeds the fixed scratch of " +
+                                          std::to_string(capElems) + " doubles"};
+    // partials + merged output: usplit * outSize (+ outSize when usplit > 1)
+    auto fits = [&](uint64_t us) { return !fixedScratch || (us > 1 ? us + 1 : 1) * outSize <= capElems; };
     uint64_t usplit = 1;
-    while (usplit * 2 <= nLoop && nGroups * usplit < 4096 && usplit * 2 * outSize <= (32ull << 20))
+    while (usplit * 2 <= nLoop && nGroups * usplit < 4096 && usplit * 2 * outSize <= (32ull << 20) &&
+           fits(usplit * 2))
         usplit *= 2;
     a.usplit = (int)usplit;
     a.loopPerSplit = nLoop / usplit;
 
-    double* part = (double*)ctx_scratch(ctx, (usplit + 1) * outSize * sizeof(double));
+    double* part = fixedScratch ? fixedScratch
+                                : (double*)ctx_scratch(ctx, (usplit + 1) * outSize * sizeof(double));
     double* fin = part + usplit * outSize;
     const unsigned blocks = (unsigned)(nGroups * usplit);
     const unsigned threads = 1u << TB;
-    const int pid = prof_begin(ctx, KK_REDUCE);
+    // event timing creates events: not on the fixed-scratch path
+    const int pid = fixedScratch ? -1 : prof_begin(ctx, KK_REDUCE);
     if (ctx->prec == CVDV_C128)
         hipLaunchKernelGGL(k_marginal<double2>, dim3(blocks), dim3(threads), 0, ctx->stream,
                            (const double2*)ctx->state(), a, part);
@@ -202,7 +218,7 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
         hipLaunchKernelGGL(k_marginal<float2>, dim3(blocks), dim3(threads), 0, ctx->stream,
                            (const float2*)ctx->state(), a, part);
     HIPCHK(hipGetLastError());
-    prof_end(ctx, pid);
+    if (pid >= 0) prof_end(ctx, pid);
     const double* res = part;
     if (usplit > 1) {
         if (usplit >= 256 && outSize <= 4096) {
@@ -219,6 +235,36 @@ void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut
     }
     HIPCHK(hipMemcpyAsync(hostOut, res, outSize * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHK(hipStreamSynchronize(ctx->stream));
+}
+
+void marginal_probs(CVDVContext* ctx, const int* regs, int nsel, double* hostOut) {
+    marginal_core(ctx, regs, nsel, hostOut, nullptr, 0);
+}
+
+// Fixed scratch of the allocation-free marginal (cvdvMeasureMultipleNoAlloc):
+// every split partial and the merged bins of one call.  Each GPU the library
+// runs on holds its own copy until the process ends.  Every entry a call reads
+// is written by that call first (each split writes every bin), so nothing needs
+// clearing between calls.
+#define CVDV_NOALLOC_SCRATCH_DOUBLES (1u << 20)  // 8 MiB
+static __device__ __align__(256) double scratch[CVDV_NOALLOC_SCRATCH_DOUBLES];
+
+// Device address of `scratch` on the current device, looked up once per device
+// (the host-side name of a __device__ array is not a device address).
+static double* noalloc_scratch() {
+    static std::mutex mu;
+    static std::map<int, double*> ptrs;
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = ptrs.find(dev);
+    if (it != ptrs.end()) return it->second;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        throw CvdvError{CVDV_ERR_HIP, std::string("hipGetSymbolAddress(scratch) failed: ") + hipGetErrorString(e)};
+    ptrs[dev] = (double*)ptr;
+    return (double*)ptr;
 }
 
 // ---------------------------------------------------------------- overlaps
@@ -371,6 +417,30 @@ void cvdvMeasureMultiple(CVDVContext* ctx, const int* regIdxs, int numRegs, doub
         seen |= 1ull << regIdxs[i];
     }
     CVDV_TRY marginal_probs(ctx, regIdxs, numRegs, probsOut); CVDV_CATCH()
+}
+
+void cvdvMeasureMultipleNoAlloc(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut) {
+    if (!ctx || !regIdxs || numRegs <= 0 || !probsOut || refuse_recording(ctx)) return;
+    if (!ctx->initialized) {
+        cvdv_set_error(CVDV_ERR_STATE, "State not initialized");
+        return;
+    }
+    uint64_t seen = 0;
+    for (int i = 0; i < numRegs; i++) {
+        if (regIdxs[i] < 0 || regIdxs[i] >= ctx->numReg) {
+            cvdv_set_error(CVDV_ERR_ARG, "Invalid register index: " + std::to_string(regIdxs[i]));
+            return;
+        }
+        if (seen & (1ull << regIdxs[i])) {
+            cvdv_set_error(CVDV_ERR_ARG, "Register listed twice: " + std::to_string(regIdxs[i]));
+            return;
+        }
+        seen |= 1ull << regIdxs[i];
+    }
+    CVDV_TRY
+    double* ptr = noalloc_scratch();
+    marginal_core(ctx, regIdxs, numRegs, probsOut, ptr, CVDV_NOALLOC_SCRATCH_DOUBLES);
+    CVDV_CATCH()
 }
 
 double cvdvGetNorm(CVDVContext* ctx) {

@@ -58,6 +58,7 @@ SIGNATURES = {
     "cvdvSetFreeMemoryOverride": ([c_void_p, c_size_t], None),
     "cvdvMeasureMultiple": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
     "cvdvMeasureMultipleCT": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
+    "cvdvMeasureMultipleNoAlloc": ([c_void_p, POINTER(c_int), c_int, POINTER(c_double)], None),
     "cvdvGetState": ([c_void_p, POINTER(c_double), POINTER(c_double)], None),
     "cvdvGetNorm": ([c_void_p], c_double),
     "cvdvGetFidelity": ([c_void_p, POINTER(c_void_p), c_int, POINTER(c_double)], None),

@@ -201,6 +201,19 @@ void   cvdvBatchMeasure(CVDVBatch* batch, int regIdx, double* probsOut);
 void   cvdvBatchNorms(CVDVBatch* batch, double* normsOut);
 void   cvdvBatchGetState(CVDVBatch* batch, int i, double* realOut, double* imagOut);
 
+/* ---- allocation-free marginal (addition; no counterpart in the reference) ----
+ * For callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * output and errors as cvdvMeasureMultiple, and the same deterministic
+ * compensated reduction, but the per-split partials live in a fixed 8 MiB
+ * device array of the library (one per GPU, held until the process ends):
+ * no hipMalloc, hipFree or allocating library call.  Marginals of more than
+ * 2^20 bins are refused (CVDV_ERR_ARG); fewer splits may be used than by
+ * cvdvMeasureMultiple, so results can differ from it by FP64 rounding.  Calls
+ * that overlap on one GPU (several host threads, several contexts) share the
+ * array and race: serialise them. */
+void   cvdvMeasureMultipleNoAlloc(CVDVContext* ctx, const int* regIdxs, int numRegs, double* probsOut);
+
 /* ---- copy-bandwidth probe (addition; measurement only) ----
  * Times a plain copy of one 2^28-amplitude complex64 state (2 GiB read + 2 GiB
  * written, buffers allocated and freed inside) with the access pattern of one

@@ -11,9 +11,15 @@
 // Registers that are adjacent in the state and keep their order in the output
 // (or are both reduced) merge into one tensor mode, so the tensor rank is what
 // the selection needs and no more.
+//
+// hipTensor is opened with dlopen on the first call, not linked: its load-time
+// initialisation ends the process on a machine without a GPU, and the library
+// must load there (builds, ABI checks).
 
+#include <dlfcn.h>
 #include <hiptensor/hiptensor.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +36,48 @@ __global__ void k_abs2(double* __restrict__ out, const C* __restrict__ st, uint6
 
 namespace {
 
+// The hipTensor entry points this file calls, resolved once per process.
+struct HtApi {
+#define CVDV_HT_FN(name) decltype(&::hiptensor##name) name = nullptr;
+    CVDV_HT_FN(Create) CVDV_HT_FN(Destroy) CVDV_HT_FN(CreateTensorDescriptor) CVDV_HT_FN(DestroyTensorDescriptor)
+    CVDV_HT_FN(CreateReduction) CVDV_HT_FN(DestroyOperationDescriptor) CVDV_HT_FN(CreatePlanPreference)
+    CVDV_HT_FN(DestroyPlanPreference) CVDV_HT_FN(EstimateWorkspaceSize) CVDV_HT_FN(CreatePlan)
+    CVDV_HT_FN(DestroyPlan) CVDV_HT_FN(Reduce) CVDV_HT_FN(GetErrorString)
+#undef CVDV_HT_FN
+};
+
+const HtApi& ht() {
+    static std::once_flag once;
+    static HtApi api;
+    static std::string err;
+    std::call_once(once, [] {
+        void* lib = dlopen("libhiptensor.so.0", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) lib = dlopen("/opt/rocm/lib/libhiptensor.so.0", RTLD_NOW | RTLD_LOCAL);
+        if (!lib) {
+            const char* e = dlerror();
+            err = std::string("cannot open libhiptensor: ") + (e ? e : "unknown error");
+            return;
+        }
+        HtApi a;
+        bool ok = true;
+#define CVDV_HT_SYM(name)                                                          \
+    a.name = reinterpret_cast<decltype(a.name)>(dlsym(lib, "hiptensor" #name)); \
+    ok = ok && a.name != nullptr;
+        CVDV_HT_SYM(Create) CVDV_HT_SYM(Destroy) CVDV_HT_SYM(CreateTensorDescriptor)
+        CVDV_HT_SYM(DestroyTensorDescriptor) CVDV_HT_SYM(CreateReduction) CVDV_HT_SYM(DestroyOperationDescriptor)
+        CVDV_HT_SYM(CreatePlanPreference) CVDV_HT_SYM(DestroyPlanPreference) CVDV_HT_SYM(EstimateWorkspaceSize)
+        CVDV_HT_SYM(CreatePlan) CVDV_HT_SYM(DestroyPlan) CVDV_HT_SYM(Reduce) CVDV_HT_SYM(GetErrorString)
+#undef CVDV_HT_SYM
+        if (!ok) {
+            err = "libhiptensor lacks an entry point the comparator needs";
+            return;
+        }
+        api = a;
+    });
+    if (!api.Reduce) throw CvdvError{CVDV_ERR_UNSUPPORTED, err};
+    return api;
+}
+
 struct TensorMode {
     int64_t len, inStride, outStride;  // outStride < 0: reduced
     int32_t id;
@@ -38,7 +86,7 @@ struct TensorMode {
 void ht_check(hiptensorStatus_t s, const char* what) {
     if (s != HIPTENSOR_STATUS_SUCCESS)
         throw CvdvError{s == HIPTENSOR_STATUS_NOT_SUPPORTED ? CVDV_ERR_UNSUPPORTED : CVDV_ERR_HIP,
-                        std::string("hipTensor: ") + what + " failed (" + hiptensorGetErrorString(s) + ")"};
+                        std::string("hipTensor: ") + what + " failed (" + ht().GetErrorString(s) + ")"};
 }
 
 // Owns the descriptors of one reduction; released on every path.
@@ -49,12 +97,12 @@ struct Reduction {
     hiptensorPlanPreference_t pref = nullptr;
     hiptensorPlan_t plan = nullptr;
     ~Reduction() {
-        if (plan) hiptensorDestroyPlan(plan);
-        if (pref) hiptensorDestroyPlanPreference(pref);
-        if (op) hiptensorDestroyOperationDescriptor(op);
-        if (in) hiptensorDestroyTensorDescriptor(in);
-        if (out) hiptensorDestroyTensorDescriptor(out);
-        if (h) hiptensorDestroy(h);
+        if (plan) ht().DestroyPlan(plan);
+        if (pref) ht().DestroyPlanPreference(pref);
+        if (op) ht().DestroyOperationDescriptor(op);
+        if (in) ht().DestroyTensorDescriptor(in);
+        if (out) ht().DestroyTensorDescriptor(out);
+        if (h) ht().Destroy(h);
     }
 };
 
@@ -136,22 +184,22 @@ void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, do
         double* sq = (double*)ctx_scratch(ctx, (n + outSize) * sizeof(double));
         double* res = sq + n;
         Reduction R;
-        ht_check(hiptensorCreate(&R.h), "hiptensorCreate");
-        ht_check(hiptensorCreateTensorDescriptor(R.h, &R.in, (uint32_t)inLen.size(), inLen.data(), inStr.data(),
+        ht_check(ht().Create(&R.h), "hiptensorCreate");
+        ht_check(ht().CreateTensorDescriptor(R.h, &R.in, (uint32_t)inLen.size(), inLen.data(), inStr.data(),
                                                  HIPTENSOR_R_64F, 256),
                  "input descriptor");
-        ht_check(hiptensorCreateTensorDescriptor(R.h, &R.out, (uint32_t)outLen.size(), outLen.data(),
+        ht_check(ht().CreateTensorDescriptor(R.h, &R.out, (uint32_t)outLen.size(), outLen.data(),
                                                  outStr.data(), HIPTENSOR_R_64F, 256),
                  "output descriptor");
-        ht_check(hiptensorCreateReduction(R.h, &R.op, R.in, inId.data(), HIPTENSOR_OP_IDENTITY, R.out, outId.data(),
+        ht_check(ht().CreateReduction(R.h, &R.op, R.in, inId.data(), HIPTENSOR_OP_IDENTITY, R.out, outId.data(),
                                           HIPTENSOR_OP_IDENTITY, R.out, outId.data(), HIPTENSOR_OP_ADD,
                                           HIPTENSOR_COMPUTE_DESC_64F),
                  "reduction descriptor");
-        ht_check(hiptensorCreatePlanPreference(R.h, &R.pref, HIPTENSOR_ALGO_DEFAULT, HIPTENSOR_JIT_MODE_NONE),
+        ht_check(ht().CreatePlanPreference(R.h, &R.pref, HIPTENSOR_ALGO_DEFAULT, HIPTENSOR_JIT_MODE_NONE),
                  "plan preference");
         uint64_t ws = 0;
-        ht_check(hiptensorEstimateWorkspaceSize(R.h, R.op, R.pref, HIPTENSOR_WORKSPACE_DEFAULT, &ws), "workspace size");
-        ht_check(hiptensorCreatePlan(R.h, &R.plan, R.op, R.pref, ws), "plan");
+        ht_check(ht().EstimateWorkspaceSize(R.h, R.op, R.pref, HIPTENSOR_WORKSPACE_DEFAULT, &ws), "workspace size");
+        ht_check(ht().CreatePlan(R.h, &R.plan, R.op, R.pref, ws), "plan");
         void* wsp = nullptr;
         if (ws > 0) HIPCHK(hipMalloc(&wsp, ws));
         HIPCHK(hipMemsetAsync(res, 0, outSize * sizeof(double), ctx->stream));  // C of D = a A + b C
@@ -163,7 +211,7 @@ void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, do
         hipError_t le = hipGetLastError();
         const double one = 1.0, zero = 0.0;
         hiptensorStatus_t rs = HIPTENSOR_STATUS_SUCCESS;
-        if (le == hipSuccess) rs = hiptensorReduce(R.h, R.plan, &one, sq, &zero, res, res, wsp, ws, ctx->stream);
+        if (le == hipSuccess) rs = ht().Reduce(R.h, R.plan, &one, sq, &zero, res, res, wsp, ws, ctx->stream);
         prof_end(ctx, pid);
         hipError_t se = hipStreamSynchronize(ctx->stream);
         if (wsp) (void)hipFree(wsp);

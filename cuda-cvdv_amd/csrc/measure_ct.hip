@@ -8,9 +8,12 @@
 // reduction (measure.hip) against a library one; the summation order is the
 // library's, so results agree to FP64 rounding, not bit for bit.
 //
-// Registers that are adjacent in the state and keep their order in the output
-// (or are both reduced) merge into one tensor mode, so the tensor rank is what
-// the selection needs and no more.
+// The reduction writes the kept registers in the state's own order and the
+// host permutes the (small) result into the requested order, so registers that
+// are adjacent in the state merge into one tensor mode whenever both are kept
+// or both are reduced: the tensor rank is what the selection needs and no more
+// (a joint marginal of 14 one-qubit registers next to an oscillator is a rank-2
+// reduction, whatever order it is requested in).
 //
 // hipTensor is opened with dlopen on the first call, not linked: its load-time
 // initialisation ends the process on a machine without a GPU, and the library
@@ -78,8 +81,16 @@ const HtApi& ht() {
     return api;
 }
 
+// One register of the canonical state: extent, input stride and the stride of
+// its index in the requested (column-major) output, -1 when it is reduced.
+struct RegMode {
+    int64_t len, inStride, outReq;
+};
+
+// A tensor mode: consecutive registers that are all kept or all reduced.
 struct TensorMode {
-    int64_t len, inStride, outStride;  // outStride < 0: reduced
+    int64_t len, inStride;
+    bool kept;
     int32_t id;
 };
 
@@ -106,31 +117,52 @@ struct Reduction {
     }
 };
 
-// Modes of the canonical state for a selection: registers from least to most
-// significant, each with its extent, input stride and output stride; adjacent
-// registers merge when both are reduced or their output strides continue the
-// same way as their input strides.
-std::vector<TensorMode> tensor_modes(const CVDVContext* ctx, const int* regs, int nsel, uint64_t& outSize) {
+// Registers of the canonical state for a selection, least significant first.
+std::vector<RegMode> reg_modes(const CVDVContext* ctx, const int* regs, int nsel, uint64_t& outSize) {
     std::vector<int64_t> ostride(ctx->numReg, -1);
     outSize = 1;
     for (int i = 0; i < nsel; ++i) {
         ostride[regs[i]] = (int64_t)outSize;
         outSize <<= ctx->nq[regs[i]];
     }
+    std::vector<RegMode> out;
+    for (int r = ctx->numReg - 1; r >= 0; --r)
+        if (ctx->nq[r] > 0) out.push_back(RegMode{1ll << ctx->nq[r], 1ll << ctx->flw[r], ostride[r]});
+    return out;
+}
+
+// The reduction's modes: the output keeps the kept registers in the state's own
+// order (compact strides; the host permutes afterwards), so consecutive
+// registers merge whenever both are kept or both are reduced and their input
+// strides continue.  The rank is then at most the number of alternations
+// between kept and reduced registers, plus one.
+std::vector<TensorMode> tensor_modes(const std::vector<RegMode>& regs) {
     std::vector<TensorMode> modes;
-    for (int r = ctx->numReg - 1; r >= 0; --r) {
-        if (ctx->nq[r] == 0) continue;
-        const int64_t len = 1ll << ctx->nq[r], in = 1ll << ctx->flw[r], out = ostride[r];
+    bool anyReduced = false;
+    for (const RegMode& r : regs) anyReduced = anyReduced || r.outReq < 0;
+    if (!anyReduced) {
+        // nothing to sum (every register kept): the library refuses a rank-1
+        // tensor with no reduced mode, so keep one mode per register, and split a
+        // lone register in two
+        for (const RegMode& r : regs) modes.push_back(TensorMode{r.len, r.inStride, true, (int32_t)modes.size()});
+        if (modes.size() == 1 && modes[0].len >= 4) {
+            int64_t lo = 1;
+            while (lo * lo < modes[0].len) lo *= 2;
+            const int64_t hi = modes[0].len / lo;
+            modes = {TensorMode{lo, modes[0].inStride, true, 0}, TensorMode{hi, modes[0].inStride * lo, true, 1}};
+        }
+        return modes;
+    }
+    for (const RegMode& r : regs) {
+        const bool kept = r.outReq >= 0;
         if (!modes.empty()) {
             TensorMode& m = modes.back();
-            const bool bothReduced = m.outStride < 0 && out < 0;
-            const bool continues = m.outStride >= 0 && out == m.outStride * m.len && in == m.inStride * m.len;
-            if ((bothReduced && in == m.inStride * m.len) || continues) {
-                m.len *= len;
+            if (m.kept == kept && r.inStride == m.inStride * m.len) {
+                m.len *= r.len;
                 continue;
             }
         }
-        modes.push_back(TensorMode{len, in, out, (int32_t)modes.size()});
+        modes.push_back(TensorMode{r.len, r.inStride, kept, (int32_t)modes.size()});
     }
     return modes;
 }
@@ -160,20 +192,16 @@ void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, do
     try {
         ctx_materialize(ctx);  // canonical layout, pending phases applied
         uint64_t outSize = 0;
-        const std::vector<TensorMode> modes = tensor_modes(ctx, regIdxs, numRegs, outSize);
+        const std::vector<RegMode> regs = reg_modes(ctx, regIdxs, numRegs, outSize);
+        const std::vector<TensorMode> modes = tensor_modes(regs);
         std::vector<int64_t> inLen, inStr, outLen, outStr;
         std::vector<int32_t> inId, outId;
-        // The reduction keeps the selected modes in the state's own order (a
-        // permuted output of four or more modes came back wrong from the library);
-        // the result, at most a few MiB, is then permuted on the host.
-        std::vector<const TensorMode*> kept;
         int64_t compact = 1;
         for (const TensorMode& m : modes) {
             inLen.push_back(m.len);
             inStr.push_back(m.inStride);
             inId.push_back(m.id);
-            if (m.outStride >= 0) {
-                kept.push_back(&m);
+            if (m.kept) {
                 outLen.push_back(m.len);
                 outStr.push_back(compact);
                 outId.push_back(m.id);
@@ -220,12 +248,14 @@ void cvdvMeasureMultipleCT(CVDVContext* ctx, const int* regIdxs, int numRegs, do
         HIPCHK(se);
         std::vector<double> natural(outSize);
         HIPCHK(hipMemcpy(natural.data(), res, outSize * sizeof(double), hipMemcpyDeviceToHost));
-        // natural index (kept modes, least significant first) -> requested column-major index
+        // natural index (kept registers, least significant first) -> requested
+        // column-major index
         for (uint64_t e = 0; e < outSize; ++e) {
             uint64_t rem = e, o = 0;
-            for (const TensorMode* m : kept) {
-                o += (rem % (uint64_t)m->len) * (uint64_t)m->outStride;
-                rem /= (uint64_t)m->len;
+            for (const RegMode& r : regs) {
+                if (r.outReq < 0) continue;
+                o += (rem % (uint64_t)r.len) * (uint64_t)r.outReq;
+                rem /= (uint64_t)r.len;
             }
             probsOut[o] = natural[e];
         }

@@ -96,8 +96,11 @@ def test_device_readouts_equal_host(prec):
 
 @pytest.mark.parametrize("prec", ["complex128", "mixed"])
 @pytest.mark.parametrize("regs,sel", [([2, 2, 12], [0, 1, 2]), ([2, 2, 12], [2]), ([1, 10], [0]),
-                                      ([1, 1, 1, 1, 10], [0, 1, 2, 3]), ([3, 4, 5], [2, 0])],
-                         ids=["2x2x12-012", "2x2x12-2", "1x10-0", "1x1x1x1x10-0123", "3x4x5-20"])
+                                      ([1, 1, 1, 1, 10], [0, 1, 2, 3]), ([3, 4, 5], [2, 0]),
+                                      ([1] * 6 + [8], list(range(6))), ([1] * 6 + [8], [5, 3, 1, 0, 2, 4]),
+                                      ([2, 3, 2, 6], [3, 1]), ([2, 2, 12], [2, 0, 1]), ([12], [0])],
+                         ids=["2x2x12-012", "2x2x12-2", "1x10-0", "1x1x1x1x10-0123", "3x4x5-20",
+                              "1x6+8-012345", "1x6+8-531024", "2x3x2x6-31", "2x2x12-201", "12-0"])
 def test_library_reduction_comparator(regs, sel, prec):
     """cvdvMeasureMultipleCT (one hipTensor reduction of |psi|^2) against the own
     deterministic reduction, on a random state with a lazy layout pending."""

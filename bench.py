@@ -376,17 +376,19 @@ def c5_section(steps):
                          ("husimi_512_per_branch", lambda: h.getHusimiQGrid(2, grid, grid, per_slice=True)),
                          ("wigner_native_traced", lambda: h.getWigner(2)),
                          ("husimi_native_traced", lambda: h.getHusimiQ(2)),
-                         # the same readouts into device memory (no 128 MiB copy to pageable host memory)
-                         ("wigner_native_traced_device", lambda: h.getWigner(2, device=True).cpu()),
-                         ("husimi_native_traced_device", lambda: h.getHusimiQ(2, device=True).cpu()),
+                         # the same readouts into device memory (no 128 MiB copy to pageable host
+                         # memory); the entries return once the result is complete
+                         ("wigner_native_traced_device", lambda: h.getWigner(2, device=True)),
+                         ("husimi_native_traced_device", lambda: h.getHusimiQ(2, device=True)),
                          ("joint_m012", lambda: h.m(0, 1, 2)),
                          ("joint_m012_hiptensor", lambda: _ct_marginal(h, [0, 1, 2]))):
             fn()  # untimed: first-call costs (code objects, library handles, work buffers)
             t0 = time.perf_counter()
             for _ in range(steps):
                 val = fn()
+            torch.cuda.synchronize()
             t[name + "_ms"] = (time.perf_counter() - t0) / steps * 1e3
-            outs[(prec, name)] = np.asarray(val)
+            outs[(prec, name)] = val.cpu().numpy() if torch.is_tensor(val) else np.asarray(val)
         # device readouts: the time spent in the library's kernels, for the host-side share
         h._lib.cvdvProfileEnable(h.ctx, 1)
         h.getWigner(2, device=True)
@@ -683,16 +685,33 @@ def dft_gemm_probe(reps=5):
 
 
 def _fock_table(N):
-    """Grid-sampled Hermite functions |n>, n < N, by the three-term recurrence
-    (separable.py:62-74's builder for every n at once), FP64."""
+    """Grid-sampled Hermite functions |n>, n < N (separable.py:62-74's recurrence
+    for every n at once), FP64.  The recurrence runs on mantissas m_n(x) with a
+    per-point log scale l(x) (h_n = m_n e^l, l starting at -x^2/2), renormalised
+    by 2^-500 when a mantissa passes 2^500, and each row is formed as
+    sign(m) e^(log|m| + l): e^(-x^2/2) alone underflows for x^2/2 > 745, which
+    would cut the tails of every |n> above n ~ 745 (grids of 2^11 points and up)."""
     dx = math.sqrt(2 * math.pi / N)
     x = (np.arange(N) - (N - 1) / 2) * dx
     F = np.empty((N, N))
-    F[0] = math.pi ** -0.25 * math.sqrt(dx) * np.exp(-0.5 * x * x)
+    lscale = -0.5 * x * x
+    cur = np.full(N, math.pi ** -0.25 * math.sqrt(dx))
     prev = np.zeros(N)
+
+    def row(m):
+        with np.errstate(divide="ignore"):
+            return np.sign(m) * np.exp(np.log(np.abs(m)) + lscale)
+
+    F[0] = row(cur)
     for k in range(1, N):
-        F[k] = math.sqrt(2.0 / k) * x * F[k - 1] - math.sqrt((k - 1.0) / k) * prev
-        prev = F[k - 1]
+        nxt = math.sqrt(2.0 / k) * x * cur - math.sqrt((k - 1.0) / k) * prev
+        big = np.abs(nxt) > 2.0 ** 500
+        if big.any():
+            s = np.where(big, 2.0 ** -500, 1.0)
+            nxt, cur = nxt * s, cur * s
+            lscale = lscale + np.where(big, 500 * math.log(2.0), 0.0)
+        prev, cur = cur, nxt
+        F[k] = row(cur)
     return F
 
 
@@ -707,7 +726,7 @@ def _upper_fit(ks, eps):
     return float(a), float(b)
 
 
-def qft_error_envelope(qubits=range(6, 13)):
+def qft_error_envelope(qubits=range(5, 13)):
     """Precision envelope of the register QFT (the reference's analysis/qft_err.py
     and ERROR-ANALYSIS.md:13-16, rerun on this library): for N = 2^q, every Fock
     state |n> is transformed by ftQ2P (all n at once, as the slices of an [q, q]
@@ -753,24 +772,32 @@ def qft_error_envelope(qubits=range(6, 13)):
         print(f"[bench] qft envelope N={N}: c128 floor {row['complex128']['floor']:.1e}, mixed floor "
               f"{row['mixed']['floor']:.1e}, mixed-vs-c128 {row['mixed_vs_c128_max_l2']:.1e}", file=sys.stderr,
               flush=True)
+    # a(N) = c_a N^-1/2 + d_a, b(N) = c_b N^1/2 + d_b over every N swept, and over
+    # the reference's own sweep (N = 32 .. 1024, analysis/qft_err.py:25), whose
+    # published complex128 fit is a = 4.2125 N^-1/2 + 0.1027, b = -5.7903 N^1/2 + 16.2724
     for prec, fp in fits.items():
-        if len(fp) >= 2:
-            Ns = np.array([f[0] for f in fp], dtype=float)
-            ca, da = np.polyfit(Ns ** -0.5, [f[1] for f in fp], 1)
-            cb, db = np.polyfit(Ns ** 0.5, [f[2] for f in fp], 1)
-            out["fits"][prec] = {"a(N)": [float(ca), float(da)], "b(N)": [float(cb), float(db)]}
+        for key, sel in (("all_N", fp), ("N_32_to_1024", [f for f in fp if 32 <= f[0] <= 1024])):
+            if len(sel) >= 2:
+                Ns = np.array([f[0] for f in sel], dtype=float)
+                ca, da = np.polyfit(Ns ** -0.5, [f[1] for f in sel], 1)
+                cb, db = np.polyfit(Ns ** 0.5, [f[2] for f in sel], 1)
+                out["fits"].setdefault(prec, {})[key] = {"a(N)": [float(ca), float(da)],
+                                                         "b(N)": [float(cb), float(db)]}
     return out
 
 
-def pmc_traffic(kernel):
-    """HBM bytes per launch of `kernel` from its committed counter summary
-    (profiles/<kernel>_c2_mixed_pmc.txt: FETCH_SIZE and WRITE_SIZE, each scaled
-    by the same counter on a plain copy of known size), or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_c2_mixed_pmc.txt")
+def pmc_traffic(kind):
+    """HBM bytes per launch of the FFT pass kind `kind` (fft_row, fft_half,
+    fft_fused) from the committed counter summary of the [14,14] complex64
+    Gaussian circuit (profiles/fft_c2_mixed_pmc.txt: FETCH_SIZE and WRITE_SIZE
+    per kernel, each scaled by the same counter on a plain copy of known size),
+    or None.  The counters come from separate rocprofv3 runs, not from this one."""
+    path = os.path.join(ROOT, "profiles", "fft_c2_mixed_pmc.txt")
+    key = f"traffic_bytes_per_launch[{kind}]"
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch"):
+                if line.startswith(key):
                     return float(line.split("=")[1].split()[0])
     except OSError:
         pass
@@ -798,7 +825,7 @@ def roofline(prof, pass_bytes, copy):
     alg = per * pass_bytes
     achieved = alg / (avg * 1e-3) / 1e9
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(name.split()[0]) if dom == "fft_fused" else None,
+           "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(dom),
            "kernel": name, "launches": n, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": alg,
            "copy_gbps": copy["gbps"], "frac_of_copy": achieved / copy["gbps"], "share_of_fft_ms": ms / prof["fft_ms"],
            "fft_kinds": {k: {"kernel": FFT_KINDS[k][0], "launches": prof[k + "_launches"], "ms": prof[k + "_ms"],

@@ -292,6 +292,8 @@ void twiddle_cache_free(TwiddleCache* tc);
 // completing an L half writes the other state buffer; when it does not exist
 // yet and allocation is not allowed, throws CVDV_ERR_STATE.
 void fft_settle(CVDVContext* ctx, bool mayAllocate = true);
+// A transform of field [sh, sh+nb) runs as halves and keeps one pending.
+bool fft_defers(const CVDVContext* ctx, int sh, int nb);
 // Complete the half-done transforms whose fields the program reads, outside the
 // bits `except`.
 void fft_settle_for(CVDVContext* ctx, const DiagProg& p, uint64_t except = 0);

@@ -1601,6 +1601,16 @@ static bool fft_deferred(CVDVContext* ctx, int sh, int nb, bool inverse, const D
     return true;
 }
 
+bool fft_defers(const CVDVContext* ctx, int sh, int nb) {
+    const uint64_t N = 1ull << nb, S = 1ull << sh, total = ctx->totalSize;
+    if (total < 4096 || N <= 64 || (total % (N * S)) != 0) return false;
+    PassGeom g1{(int)N, S, total / (N * S), 1, 0, N * S, S, 0, N * S, S};
+    FftPass d{};
+    if (fit_pass(g1, ctx->prec, d, true)) return false;
+    FourStep f;
+    return four_step_geom(ctx->prec, total, sh, nb, f) && fusable(ctx, f, nb);
+}
+
 // Every twiddle table an FFT of any register of the context can ask for.
 void fft_prepare(CVDVContext* ctx) {
     for (int r = 0; r < ctx->numReg; ++r) {

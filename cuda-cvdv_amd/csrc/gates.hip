@@ -901,11 +901,19 @@ static void bs_small(CVDVContext* ctx, int r1, int r2, double th, int c = -1, in
     // gates.cu:614-636 (and 640-666 with a control)
     if (std::fabs(th) < 1e-15) return;
     const double cq = -std::tan(0.25 * th), cp = -std::sin(0.5 * th);
-    // The two transforms act on different registers and commute.  The first one
-    // applies the pending q1*q2 phase; the register with the lower physical
-    // field goes first, since its rows are contiguous, where the phase reduces
-    // to one affine term per row instead of a per-element evaluation.
-    const int fa = ctx->pshift[r1] <= ctx->pshift[r2] ? r1 : r2, fb = fa == r1 ? r2 : r1;
+    // The two transforms act on different registers and commute; the first one
+    // of each pair applies the pending q1*q2 (p1*p2) phase.  A register whose
+    // transform keeps a half pending goes first: the other register's pass then
+    // finds no pending term on its bits, and its two transforms join one chain of
+    // fused passes (a pending half must be completed before any pass whose
+    // phases read its bits).  Otherwise the register with the lower physical
+    // field goes first, since its rows are contiguous, where the phase reduces to
+    // one affine term per row instead of a per-element evaluation.
+    const bool d1 = fft_defers(ctx, ctx->pshift[r1], ctx->nq[r1]);
+    const bool d2 = fft_defers(ctx, ctx->pshift[r2], ctx->nq[r2]);
+    int fa = ctx->pshift[r1] <= ctx->pshift[r2] ? r1 : r2;
+    if (d1 != d2) fa = d1 ? r1 : r2;
+    const int fb = fa == r1 ? r2 : r1;
     phase_bilin(ctx, r1, r2, cq, c, q);
     ctx_ft(ctx, fa, false);
     ctx_ft(ctx, fb, false);

@@ -537,9 +537,17 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
 // Pending phases, the four-step twiddle W_N^{j1 k2} of a second pass (row j1 =
 // the transformed index, k2 = (g >> tw4Shift) mod 2^tw4HiBits; TW4: the
 // instance may carry one) and the input conjugation of an inverse on the loaded
-// inputs.  ANYTAB: tables may need the per-element flat index (else the host
-// has checked that every table is read by row, table_by_row()).
-template <int RP, int E, int LGM, int LGC, bool TW4, bool ANYTAB, typename C>
+// inputs.  TM, how the phase tables are read:
+//   TM_ROWS   every table is read by row (the host checked, table_by_row()), one
+//             load per element at a row offset (one-row tiles);
+//   TM_ANY    tables of several columns stage their product by row in LDS when
+//             every table allows it, else each element takes its flat index;
+//   TM_STAGED every table is read by row (the host checked, tile_tables_by_row())
+//             and is staged by row; generic terms are evaluated one element per
+//             iteration of a rolled loop into the free exchange buffer, which
+//             keeps their FP64 temporaries off the other elements' registers.
+enum { TM_ROWS = 0, TM_ANY = 1, TM_STAGED = 2 };
+template <int RP, int E, int LGM, int LGC, bool TW4, int TM, typename C>
 __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
                                            const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt,
                                            C* sRow) {
@@ -648,7 +656,7 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             const DiagTab& T = A.T;
             const bool fast = A.fast;
             constexpr int CH = sizeof(C) == 16 ? (RP < 8 ? 2 : 4) : 8;  // entries in flight per batch
-            if (!ANYTAB || fast) {
+            if (TM == TM_ROWS || fast) {
                 const C* tb = A.tb;
                 const int s = A.s;
                 const bool cj = A.cj;
@@ -665,7 +673,7 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
 #pragma unroll
                     for (int j = 0; j < CH; ++j) x[j0 + j] = cmul(x[j0 + j], e[j]);
                 }
-            } else if constexpr (ANYTAB) {
+            } else if constexpr (TM == TM_ANY) {
                 // per element; the choice bits are tile-uniform (the tables were
                 // compiled for this tiling), so the choice is one per tile
                 const C* tb = (const C*)pre.arena + T.off + (T.nsel > 0 ? tab_choice(T, inBase) : 0u);
@@ -684,14 +692,15 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             }
         };
         bool staged = false;
-        if constexpr (LGC > 0 && ANYTAB) {
+        if constexpr (LGC > 0 && TM != TM_ROWS) {
             // Tiles of several columns: when every table is read by row, their
             // product over the tile's M rows is staged in LDS (one table load per
             // row instead of one per element), and each element takes one LDS
             // read and one product.  The exchange buffer is free before the
             // stages; a barrier hands it back.
             bool byRow = pre.ntab > 0;
-            for (int t = 0; t < pre.ntab; ++t) byRow = byRow && tab_at(P.tab[t]).fast;
+            if constexpr (TM == TM_ANY)
+                for (int t = 0; t < pre.ntab; ++t) byRow = byRow && tab_at(P.tab[t]).fast;
             if (byRow) {
                 for (int r = tid; r < (1 << LGM); r += nt) {
                     C f = cmk<C>(1.0, 0.0);
@@ -714,7 +723,7 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 staged = true;
             }
         }
-        if (!staged) {
+        if (TM != TM_STAGED && !staged) {
             if (pre.ntab > 0) apply_table(P.tab[0]);
             for (int t = 1; t < pre.ntab; ++t) {
                 // opaque: the element indices are recomputed per table, not held
@@ -722,7 +731,23 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 apply_table(P.tab[t]);
             }
         }
-        if (pre.ngen > 0) {
+        if constexpr (TM == TM_STAGED && LGC > 0) {
+            if (pre.ngen > 0) {
+                // one element per iteration: its factor goes to its own slot of the
+                // exchange buffer (element-major, so a wave's writes are
+                // consecutive), then every element takes its product
+#pragma unroll 1
+                for (int j = 0; j < E; ++j) {
+                    const uint64_t g = elem(j / RP, j % RP);
+                    double r = pre.genrev;
+                    for (int t = 0; t < pre.ngen; ++t) r += cvdv_frac(diag_term_rev(P.gen[t], g));
+                    sRow[j * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
+                }
+#pragma unroll
+                for (int j = 0; j < E; ++j) x[j] = cmul(x[j], sRow[j * nt + tt]);
+                lds_barrier();  // the stages that follow write other threads' slots
+            }
+        } else if (pre.ngen > 0) {
             asm volatile("" : "+v"(tt));
             // one-row tiles: the terms are affine in the row coordinate
             GenFold gf{0.0, 0.0, 0};
@@ -879,7 +904,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     auto prephase = [&](C* x, uint64_t t, int ti) {
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
-        stage0_pre<R0, E, LGM, LGC, !PF, !PF>(x, d, inBase, P, pre, tw4, ti, nt, lds);
+        stage0_pre<R0, E, LGM, LGC, !PF, PF ? TM_ROWS : TM_ANY>(x, d, inBase, P, pre, tw4, ti, nt, lds);
     };
     // Transform the phased tile in registers (the last stage's outputs stay in x).
     auto stages = [&](C* x, int ti, const C* twt) {
@@ -1004,8 +1029,9 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
 // swapN > 0 exchanges the two swapN-bit digits at swapSh of the tile base (the
 // in-place L half leaves the register's digits exchanged).  Between the halves
 // the elements go through LDS from the last stage's order to stage 0's.
-template <typename R, int LGM, int LGC, int E>
+template <typename R, int LGM, int LGC, int E, bool STG>
 __global__ void __launch_bounds__((tile_threads_max<R, LGM, LGC>()))
+__attribute__((amdgpu_waves_per_eu((STG && sizeof(R) == 4) ? 6 : 1)))
 k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int conjMid, int swapSh, int swapN,
             const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
     using C = typename CT<R>::type;
@@ -1058,7 +1084,7 @@ k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int
         for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
     }
     // pending half: the four-step twiddle (d carries no phases and no input conjugation)
-    stage0_pre<R0, E, LGM, LGC, true, false>(x, d, base, P, pre, tw4, tid, nt, lds);
+    stage0_pre<R0, E, LGM, LGC, true, TM_ROWS>(x, d, base, P, pre, tw4, tid, nt, lds);
     stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
     run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
     if (conjMid) {
@@ -1071,7 +1097,7 @@ k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int
     stage_read<R0, E, LGM, LGC>(lds, x, tid, nt);
     lds_barrier();
     // phases at the natural index, then the next transform's first half
-    stage0_pre<R0, E, LGM, LGC, false, true>(x, dD, natBase, P, pre, tw4, tid, nt, lds);
+    stage0_pre<R0, E, LGM, LGC, false, STG ? TM_STAGED : TM_ANY>(x, dD, natBase, P, pre, tw4, tid, nt, lds);
     stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
     run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
     last_store<RL, E, LGM, LGC>(data, x, d, base, scale, scaleLo, tid, nt);
@@ -1421,6 +1447,29 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
 #define FUSE_LGM_MIN 5
 #define FUSE_LGM_MAX 8
 
+// Every table of the program is read by row in tiles of geometry d (the test
+// stage0_pre makes per table, made here on the host): its field misses the
+// tile's column and row-block bits and holds all or none of the row bits, and
+// no control or choice bit of it varies inside the tile.
+static bool tile_tables_by_row(const FastDiag& pre, const FftPass& d) {
+    const int rowLo = d.lgInR, rowHi = d.lgInR + d.lgM;
+    auto hits = [](int sh, int nb, int lo, int n) { return n > 0 && sh < lo + n && lo < sh + nb; };
+    auto varies = [&](int b) {
+        return b >= 0 && (b < d.lgC || (b >= rowLo && b < rowHi) ||
+                          (d.lgHb > 0 && b >= d.lgInH2 && b < d.lgInH2 + d.lgHb));
+    };
+    for (int i = 0; i < pre.ntab; ++i) {
+        const DiagTab& T = pre.tab[i];
+        const bool rowIn = T.sh <= rowLo && rowHi <= T.sh + T.nb;
+        bool ctrlVar = varies(T.ctrl);
+        for (int j = 0; j < T.nsel; ++j) ctrlVar = ctrlVar || varies(tab_sel(T, j));
+        const bool fast = !hits(T.sh, T.nb, 0, d.lgC) && !hits(T.sh, T.nb, d.lgInH2, d.lgHb) &&
+                          (rowIn || !hits(T.sh, T.nb, rowLo, d.lgM)) && !ctrlVar;
+        if (!fast) return false;
+    }
+    return true;
+}
+
 template <typename R, int LGM, int LGC>
 static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, const FftPass& d, const FftPass& dD,
                               const FastDiag& pre, int conjMid, int swapSh, int swapN,
@@ -1429,8 +1478,18 @@ static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, cons
     if constexpr (tile_instance<R, LGM, LGC>()) {
         if (tile / 8 > tile_threads_max<R, LGM, LGC>())
             throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile larger than its kernel's workgroup bound"};
-        hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8>), dim3((unsigned)blocks), dim3(tile / 8), lds, ctx->stream,
-                           data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
+        // tiles of several columns whose tables are all read by row take the
+        // instance that stages them (and the generic terms) through LDS: fewer
+        // registers, more workgroups per CU
+        if constexpr (LGC > 0) {
+            if (!dD.usePre || tile_tables_by_row(pre, dD)) {
+                hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8, true>), dim3((unsigned)blocks), dim3(tile / 8), lds,
+                                   ctx->stream, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
+                return;
+            }
+        }
+        hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8, false>), dim3((unsigned)blocks), dim3(tile / 8), lds,
+                           ctx->stream, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
     } else {
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile shape without a kernel instance"};
     }

@@ -787,15 +787,22 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
     }
 }
 
-// Output conjugation (inverse) and scale.
-template <typename R, typename C>
-__device__ __forceinline__ C out_finish(C v, const FftPass& d, R scale, R scaleLo) {
-    if (d.conjOut) v.y = -v.y;
+// Output scale and conjugation (inverse), applied to a thread's transformed
+// elements before either store path.  Behind uniform branches: the compiler
+// would otherwise select per element and pay for the scale even when it is off.
+template <int E, typename R, typename C>
+__device__ __forceinline__ void out_finish(C* x, const FftPass& d, R scale, R scaleLo) {
     if (d.useScale) {
-        v.x = fma(v.x, scale, v.x * scaleLo);
-        v.y = fma(v.y, scale, v.y * scaleLo);
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            x[i].x = fma(x[i].x, scale, x[i].x * scaleLo);
+            x[i].y = fma(x[i].y, scale, x[i].y * scaleLo);
+        }
     }
-    return v;
+    if (d.conjOut) {
+#pragma unroll
+        for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
+    }
 }
 
 // Last-stage outputs straight to global memory (rows q + m*M/RP of column c).
@@ -812,7 +819,7 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* v, cons
 #pragma unroll
         for (int m = 0; m < RP; ++m) {
             const int row = p.q + (m << lgMq);
-            out[g0 + ((uint64_t)row << d.lgOutR)] = out_finish(v[k * RP + m], d, scale, scaleLo);
+            out[g0 + ((uint64_t)row << d.lgOutR)] = v[k * RP + m];
         }
     }
 }
@@ -922,7 +929,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         uint64_t lo0, inBase, outBase;
         tile_base(t, lo0, inBase, outBase);
         auto value = [&](int c, int row, int hb) {
-            return out_finish(lds[lds_ix((((hb << LGM) + row) << lgC) + c)], d, scale, scaleLo);
+            return lds[lds_ix((((hb << LGM) + row) << lgC) + c)];
         };
         if (sizeof(C) == 8 && d.vecOut) {
 #pragma unroll
@@ -947,6 +954,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         }
     };
     auto store = [&](C* x, uint64_t t, int ti) {
+        out_finish<E>(x, d, scale, scaleLo);
         if (d.directOut) {
             uint64_t lo0, inBase, outBase;
             tile_base(t, lo0, inBase, outBase);
@@ -1100,6 +1108,7 @@ k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int
     stage0_pre<R0, E, LGM, LGC, false, STG ? TM_STAGED : TM_ANY>(x, dD, natBase, P, pre, tw4, tid, nt, lds);
     stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
     run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
+    out_finish<E>(x, d, scale, scaleLo);
     last_store<RL, E, LGM, LGC>(data, x, d, base, scale, scaleLo, tid, nt);
 }
 

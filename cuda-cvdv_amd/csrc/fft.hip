@@ -518,6 +518,32 @@ __device__ __forceinline__ C gen_factor(const PreLds& P, const FastDiag& pre, ui
     return unit_rev_fast(cvdv_frac(r), R(0));
 }
 
+// Every generic term is affine in the row index of a tile whose rows are the
+// bits [rowLo, rowLo + rowN) (column and row-block bits fixed): each field holds
+// all of the row bits or none (a bilinear term at most one field with them, a
+// polynomial with them only its linear coefficient), and no control or Z bit
+// is a row bit.  Uniform across the tile.
+__device__ __forceinline__ bool gen_affine_in_rows(const PreLds& P, const FastDiag& pre, int rowLo, int rowN) {
+    auto rel = [&](int sh, int nb) {  // 0: misses the rows, 1: holds them all, 2: part of them
+        if (!(sh < rowLo + rowN && rowLo < sh + nb)) return 0;
+        return (sh <= rowLo && rowLo + rowN <= sh + nb) ? 1 : 2;
+    };
+    for (int i = 0; i < pre.ngen; ++i) {
+        const DiagTerm& T = P.gen[i];
+        if (T.ctrl >= rowLo && T.ctrl < rowLo + rowN) return false;
+        if (T.kind == DT_POLY) {
+            const int r = rel(T.sh1, T.nb1);
+            if (r == 2 || (r == 1 && (T.c2 != 0.0 || T.c3 != 0.0))) return false;
+        } else if (T.kind == DT_BILIN) {
+            const int r1 = rel(T.sh1, T.nb1), r2 = rel(T.sh2, T.nb2);
+            if (r1 == 2 || r2 == 2 || (r1 == 1 && r2 == 1)) return false;
+        } else if (T.sh1 >= rowLo && T.sh1 < rowLo + rowN) {
+            return false;
+        }
+    }
+    return true;
+}
+
 // Stage-0 inputs straight from global memory (issued early: a tile ahead).
 template <int RP, int E, int LGM, int LGC, typename C>
 __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
@@ -733,15 +759,37 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         }
         if constexpr (TM == TM_STAGED && LGC > 0) {
             if (pre.ngen > 0) {
-                // one element per iteration: its factor goes to its own slot of the
+                // One element per iteration: its factor goes to its own slot of the
                 // exchange buffer (element-major, so a wave's writes are
-                // consecutive), then every element takes its product
+                // consecutive), then every element takes its product.  complex64,
+                // every generic term affine in the row index: a butterfly's rows are
+                // q + i M/RP (column and row block fixed), so its phases are P0 + i D
+                // from two exact FP64 evaluations per butterfly.
+                const bool aff = sizeof(C) == 8 && gen_affine_in_rows(P, pre, d.lgInR, LGM);
 #pragma unroll 1
-                for (int j = 0; j < E; ++j) {
-                    const uint64_t g = elem(j / RP, j % RP);
-                    double r = pre.genrev;
-                    for (int t = 0; t < pre.ngen; ++t) r += cvdv_frac(diag_term_rev(P.gen[t], g));
-                    sRow[j * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
+                for (int k = 0; k < NB; ++k) {
+                    double p0 = pre.genrev, dl = 0.0;
+                    if (aff) {
+                        const uint64_t g0 = elem(k, 0), g1 = elem(k, 1);
+                        for (int t = 0; t < pre.ngen; ++t) {
+                            const double v0 = diag_term_rev(P.gen[t], g0);
+                            p0 += cvdv_frac(v0);
+                            dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - v0);
+                        }
+                        dl = cvdv_frac(dl);
+                    }
+#pragma unroll 1
+                    for (int i = 0; i < RP; ++i) {
+                        double r;
+                        if (aff) {
+                            r = fma((double)i, dl, p0);
+                        } else {
+                            const uint64_t g = elem(k, i);
+                            r = pre.genrev;
+                            for (int t = 0; t < pre.ngen; ++t) r += cvdv_frac(diag_term_rev(P.gen[t], g));
+                        }
+                        sRow[(k * RP + i) * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
+                    }
                 }
 #pragma unroll
                 for (int j = 0; j < E; ++j) x[j] = cmul(x[j], sRow[j * nt + tt]);

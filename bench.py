@@ -480,10 +480,12 @@ def c1_section(steps):
             "max_abs_err_m0": float(np.max(np.abs(np.asarray(m0) - np.asarray(mo))))}
 
 
-def run_gpu(prec, ops, steps, warmup, fuse=True):
+def run_gpu(prec, ops, steps, warmup, fuse=True, tune=True):
     """Time `steps` runs of the circuit on one context.  The timed region ends with
     cvdvFlush, which completes a half-done transform, applies the pending phases
-    and restores the canonical layout: every gate of every step has executed."""
+    and restores the canonical layout: every gate of every step has executed.
+    With `tune`, the FFT autotuner picks the tile plans first (outside the timed
+    region; with fusion off it tunes tile shapes only)."""
     import ctypes
 
     import torch
@@ -495,6 +497,14 @@ def run_gpu(prec, ops, steps, warmup, fuse=True):
     sep.setCoherent(0, 1.2 - 0.4j)
     sep.setCoeffs(1, torch.from_numpy(squeezed_vacuum(REGS[1], 0.5)))
     h.initStateVector(sep)
+    tuning = None
+    if tune:
+        t0 = time.perf_counter()
+        tuning = h.autotune()
+        if not fuse:  # the tuner may pick fusion for a field: keep every transform whole
+            h._c("cvdvSetTransformFusion", ctypes.c_int(0))
+        tuning["seconds"] = time.perf_counter() - t0
+        print(f"[bench] {prec}: FFT autotune {tuning}", file=sys.stderr, flush=True)
 
     def step():
         for op in ops:
@@ -526,6 +536,7 @@ def run_gpu(prec, ops, steps, warmup, fuse=True):
     prof["fft_half_launches"] = prof["fft_launches"] - prof["fft_fused_launches"] - prof["fft_row_launches"]
     prof["fft_half_ms"] = prof["fft_ms"] - prof["fft_fused_ms"] - prof["fft_row_ms"]
     h._lib.cvdvProfileEnable(h.ctx, 0)
+    prof["tuning"] = tuning
     return h, el, prof
 
 
@@ -922,6 +933,7 @@ def main():
                                "(d, r, s, bs 50:50, bs random), seed 2",
                    "amplitudes": A, "passes_per_step": passes,
                    "fused_transforms": not args.no_fuse,
+                   "fft_autotune": {str(k): v for k, v in (r["prof"].get("tuning") or {}).items()},
                    "fft_launches_per_step": r["prof"]["fft_launches"] / args.steps,
                    "precision": "complex64 amplitudes; FP64 angles, twiddles, sums" if main_prec == "mixed"
                    else "complex128"},

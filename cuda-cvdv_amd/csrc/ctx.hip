@@ -370,6 +370,18 @@ static CVDVContext* create_impl(int numReg, const int* numQubits, int precision,
     return ctx;
 }
 
+CVDVContext* ctx_create_like(const CVDVContext* src) {
+    CVDVContext* ctx = create_impl(src->numReg, src->nq, src->prec);
+    if (ctx->totalSize != src->totalSize) {
+        cvdvDestroy(ctx);
+        throw CvdvError{CVDV_ERR_STATE, "cannot create a context of the same shape"};
+    }
+    HIPCHK(hipMalloc(&ctx->buf[0], ctx->totalSize * ctx->elemBytes()));
+    HIPCHK(hipMemsetAsync(ctx->buf[0], 0, ctx->totalSize * ctx->elemBytes(), ctx->stream));
+    ctx->initialized = true;
+    return ctx;
+}
+
 CVDVContext* ctx_create_recording(int numReg, const int* numQubits, int precision) {
     CVDVContext* ctx = create_impl(numReg, numQubits, precision, true);
     if (!ctx->rec) {  // invalid shape: the error is recorded

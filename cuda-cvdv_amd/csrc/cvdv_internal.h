@@ -25,6 +25,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -192,6 +193,15 @@ struct FtDefer {
     double scale = 1.0;   // its normalisation, not yet applied
 };
 
+// Tile choices of the FFT of one register field (fft.hip), as cvdvFftAutotune
+// sets them; zero / negative members keep the planner's defaults.
+struct FftCfg {
+    int rowElems = 0;  // columns of a strided tile (default: 256 bytes of them)
+    int tileCap = 0;   // elements a tile grows to over row blocks (default 2048)
+    int fuse = -1;     // 0: this field's transforms run whole (no pending halves)
+};
+inline uint64_t fft_cfg_key(int sh, int nb) { return ((uint64_t)(uint32_t)sh << 8) | (uint32_t)nb; }
+
 // Every register field with a half-done transform (at most one record per field).
 struct FtDefers {
     FtDefer f[CVDV_MAX_REG];
@@ -237,6 +247,7 @@ struct CVDVContext {
     DiagProg pending{};
     FtDefers defer{};                 // half-done register transforms (fft.hip)
     int fuseFt = 1;                   // 0: every transform runs whole (no deferred halves)
+    std::map<uint64_t, FftCfg> fftCfg;  // tuned tile choices per register field (fft_cfg_key)
     size_t freeMemOverride = 0;       // readout route choice: assumed free device bytes (0 = ask)
     TwiddleCache* tw = nullptr;
     void* scratch = nullptr;          // readout scratch (grown on demand)
@@ -294,6 +305,13 @@ void twiddle_cache_free(TwiddleCache* tc);
 void fft_settle(CVDVContext* ctx, bool mayAllocate = true);
 // A transform of field [sh, sh+nb) runs as halves and keeps one pending.
 bool fft_defers(const CVDVContext* ctx, int sh, int nb);
+// Tile geometry the planner picks for field [sh, sh+nb) of the context's state
+// under cfg, folded into one number (equal numbers: the same passes); 0 when
+// the FFT of that field takes no tile pass (direct DFT of a tiny state).
+uint64_t fft_plan_signature(const CVDVContext* ctx, int sh, int nb, const FftCfg& cfg);
+// A context of src's shape and precision in the canonical layout, its state
+// zeroed, nothing pending (the FFT autotuner times candidates on it).
+CVDVContext* ctx_create_like(const CVDVContext* src);
 // Complete the half-done transforms whose fields the program reads, outside the
 // bits `except`.
 void fft_settle_for(CVDVContext* ctx, const DiagProg& p, uint64_t except = 0);

@@ -1201,12 +1201,16 @@ static uint64_t tile_bits(const FftPass& d) {
 static size_t lds_bytes(uint64_t tile, size_t esz) { return (tile + tile / 32) * esz; }
 
 // Tile limits: a strided tile stays within 64 KiB of LDS so several workgroups
-// share a CU; a whole contiguous register may take one large workgroup.
-static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
+// share a CU; a whole contiguous register may take one large workgroup.  cfg
+// (cvdvFftAutotune) may narrow the rows and change how far a tile grows over
+// row blocks.
+static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig, const FftCfg& cfg) {
     const size_t esz = prec == CVDV_C128 ? 16 : 8;
     // 256 contiguous bytes per row: the measured HBM rate of strided tiles rises
     // from ~4.0 TB/s with 128-byte rows to ~5.1 TB/s with 256-byte rows
-    const uint64_t W = prec == CVDV_C128 ? 16 : 32;
+    const uint64_t Wmax = prec == CVDV_C128 ? 16 : 32;
+    const uint64_t W = cfg.rowElems > 0 && (uint64_t)cfg.rowElems < Wmax ? (uint64_t)cfg.rowElems : Wmax;
+    const uint64_t cap = cfg.tileCap > 0 ? (uint64_t)cfg.tileCap : 2048;
     uint64_t C = g.nLo < W ? g.nLo : W;
     const size_t small = 64 * 1024;
     // the passes of a two-step split narrow their rows before giving up (a tall
@@ -1217,7 +1221,7 @@ static bool fit_pass(const PassGeom& g, int prec, FftPass& d, bool allowBig) {
     const bool big = allowBig && C == 1 && lds_bytes(tileCM, esz) <= device_max_lds() && tileCM <= 16384;
     if (lds_bytes(tileCM, esz) > small && !big) return false;
     uint64_t Hb = 1;
-    while (Hb * 2 <= g.nH2 && tileCM * Hb * 2 <= 2048) Hb *= 2;
+    while (Hb * 2 <= g.nH2 && tileCM * Hb * 2 <= cap && lds_bytes(tileCM * Hb * 2, esz) <= small) Hb *= 2;
     if (tileCM * Hb < 8) return false;
     const uint64_t tile = tileCM * Hb;
     d.lgM = ilog2(g.M);
@@ -1387,7 +1391,7 @@ struct FourStep {
     bool hasL;
 };
 
-static bool four_step_geom(int prec, uint64_t total, int sh, int nb, FourStep& f) {
+static bool four_step_geom(int prec, uint64_t total, int sh, int nb, FourStep& f, const FftCfg& cfg) {
     const uint64_t N = 1ull << nb, S = 1ull << sh;
     const uint64_t H = total / (N * S);
     if (H == 0 || (H & (H - 1)) != 0 || H * N * S != total) return false;
@@ -1403,7 +1407,7 @@ static bool four_step_geom(int prec, uint64_t total, int sh, int nb, FourStep& f
     };
     PassGeom p1{(int)N2, N1 * S, 1, H, N * S, 0, N1 * S, N * S, 0, N1 * S};
     PassGeom p2{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, S, N2 * S};
-    if (!fit_pass(p1, prec, f.d1, false) || !fit_pass(p2, prec, f.d2, false)) return false;
+    if (!fit_pass(p1, prec, f.d1, false, cfg) || !fit_pass(p2, prec, f.d2, false, cfg)) return false;
     f.d1.tw4 = 0; f.d1.tw4Shift = 0; f.d1.tw4HiBits = 0; f.d1.tw4LogN = 0; f.d1.tw4Uni = 1;
     f.d2.tw4 = 1; f.d2.tw4Shift = sh + n1; f.d2.tw4HiBits = n2; f.d2.tw4LogN = nb;
     f.d2.tw4Uni = uni(f.d2, sh + n1, n2);
@@ -1412,11 +1416,19 @@ static bool four_step_geom(int prec, uint64_t total, int sh, int nb, FourStep& f
     f.d1c.tw4Uni = uni(f.d1c, sh, n1);
     if (n1 == n2) {
         PassGeom pL{(int)N1, S, N2, H, N * S, N1 * S, S, N * S, N1 * S, S};
-        f.hasL = fit_pass(pL, prec, f.dL, false);
+        f.hasL = fit_pass(pL, prec, f.dL, false, cfg);
         f.dL.tw4 = 1; f.dL.tw4Shift = sh + n1; f.dL.tw4HiBits = n2; f.dL.tw4LogN = nb;
         f.dL.tw4Uni = uni(f.dL, sh + n1, n2);
     }
     return true;
+}
+
+// Tile choices for field [sh, sh+nb) of a buffer of `total` elements: the
+// context's tuned ones for its own state, the defaults for any other buffer.
+static FftCfg field_cfg(const CVDVContext* ctx, uint64_t total, int sh, int nb) {
+    if (total != ctx->totalSize) return FftCfg{};
+    const auto it = ctx->fftCfg.find(fft_cfg_key(sh, nb));
+    return it == ctx->fftCfg.end() ? FftCfg{} : it->second;
 }
 
 // Runs the FFT over field [sh, sh+nb) of a buffer of `total` elements.
@@ -1431,6 +1443,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
         throw CvdvError{CVDV_ERR_ARG, "FFT over a buffer whose size is not a power-of-two multiple of the field"};
     bool act = false, folded = false;
     const int inv = inverse ? 1 : 0;
+    const FftCfg cfg = field_cfg(ctx, total, sh, nb);
     // The pending phases are compiled once the tiling of the pass that applies
     // them is known: their tables take no choice bit that varies within a tile.
     auto compile_pre = [&](uint64_t tileBits) {
@@ -1441,7 +1454,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     // single pass: rows j (stride S), lo in [0,S), hi blocks in h2 (stride N*S)
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
     FftPass d{};
-    if (fit_pass(g1, prec, d, true)) {
+    if (fit_pass(g1, prec, d, true, cfg)) {
         const FastDiag pre = compile_pre(tile_bits(d));
         const int usePre = act ? 1 : 0;
         d.tw4 = 0; d.tw4Shift = 0; d.tw4HiBits = 0; d.tw4LogN = 0;
@@ -1474,7 +1487,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
         return true;
     }
     FourStep f;
-    if (!four_step_geom(prec, total, sh, nb, f))
+    if (!four_step_geom(prec, total, sh, nb, f, cfg))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
     FftPass d1 = f.d1, d2 = f.d2;
     const FastDiag pre = compile_pre(tile_bits(d1));
@@ -1595,8 +1608,8 @@ static uint64_t bit_field(int sh, int n) { return n > 0 ? (((1ull << n) - 1ull) 
 
 // The four-step of this register can leave halves pending: equal digits, fused
 // instances for the digit length, and tiles that store straight to global memory.
-static bool fusable(const CVDVContext* ctx, const FourStep& f, int nb) {
-    return ctx->fuseFt && !ctx->rec && f.n1 == f.n2 && f.n1 >= FUSE_LGM_MIN && f.n1 <= FUSE_LGM_MAX && f.hasL &&
+static bool fusable(const CVDVContext* ctx, const FourStep& f, int nb, const FftCfg& cfg) {
+    return ctx->fuseFt && cfg.fuse != 0 && !ctx->rec && f.n1 == f.n2 && f.n1 >= FUSE_LGM_MIN && f.n1 <= FUSE_LGM_MAX && f.hasL &&
            f.d1.directOut && f.dL.directOut && nb == 2 * f.n1;
 }
 
@@ -1605,7 +1618,7 @@ template <typename R>
 static void settle_impl(CVDVContext* ctx, const FtDefer D) {
     using C = typename CT<R>::type;
     FourStep f;
-    if (!four_step_geom(ctx->prec, ctx->totalSize, D.sh, D.nb, f))
+    if (!four_step_geom(ctx->prec, ctx->totalSize, D.sh, D.nb, f, field_cfg(ctx, ctx->totalSize, D.sh, D.nb)))
         throw CvdvError{CVDV_ERR_STATE, "half-done register transform without its four-step geometry"};
     C* data = (C*)ctx->state();
     if (D.high) {
@@ -1663,13 +1676,14 @@ static bool fft_deferred(CVDVContext* ctx, int sh, int nb, bool inverse, const D
     // only where the ordinary path takes the four-step split (fft_run)
     const uint64_t N = 1ull << nb, S = 1ull << sh, total = ctx->totalSize;
     if (total < 4096 || N <= 64 || (total % (N * S)) != 0) return false;
+    const FftCfg cfg = field_cfg(ctx, total, sh, nb);
     {
         PassGeom g1{(int)N, S, total / (N * S), 1, 0, N * S, S, 0, N * S, S};
         FftPass d{};
-        if (fit_pass(g1, ctx->prec, d, true)) return false;
+        if (fit_pass(g1, ctx->prec, d, true, cfg)) return false;
     }
     FourStep f;
-    if (!four_step_geom(ctx->prec, total, sh, nb, f) || !fusable(ctx, f, nb)) return false;
+    if (!four_step_geom(ctx->prec, total, sh, nb, f, cfg) || !fusable(ctx, f, nb, cfg)) return false;
     const int n = f.n1;
     bool act = false, folded = false;
     if (FtDefer* rec = ctx->defer.find(sh)) {
@@ -1720,11 +1734,26 @@ static bool fft_deferred(CVDVContext* ctx, int sh, int nb, bool inverse, const D
 bool fft_defers(const CVDVContext* ctx, int sh, int nb) {
     const uint64_t N = 1ull << nb, S = 1ull << sh, total = ctx->totalSize;
     if (total < 4096 || N <= 64 || (total % (N * S)) != 0) return false;
+    const FftCfg cfg = field_cfg(ctx, total, sh, nb);
     PassGeom g1{(int)N, S, total / (N * S), 1, 0, N * S, S, 0, N * S, S};
     FftPass d{};
-    if (fit_pass(g1, ctx->prec, d, true)) return false;
+    if (fit_pass(g1, ctx->prec, d, true, cfg)) return false;
     FourStep f;
-    return four_step_geom(ctx->prec, total, sh, nb, f) && fusable(ctx, f, nb);
+    return four_step_geom(ctx->prec, total, sh, nb, f, cfg) && fusable(ctx, f, nb, cfg);
+}
+
+uint64_t fft_plan_signature(const CVDVContext* ctx, int sh, int nb, const FftCfg& cfg) {
+    const uint64_t N = 1ull << nb, S = 1ull << sh, total = ctx->totalSize;
+    if (total % (N * S) != 0) return 0;
+    auto sig = [](const FftPass& d) { return (uint64_t)(d.lgC * 64 + d.lgHb * 4 + d.lgM / 8 + 1); };
+    PassGeom g1{(int)N, S, total / (N * S), 1, 0, N * S, S, 0, N * S, S};
+    FftPass d{};
+    if (fit_pass(g1, ctx->prec, d, true, cfg)) return 1000000 + sig(d);
+    if (total < 4096 || N <= 64) return 0;
+    FourStep f;
+    if (!four_step_geom(ctx->prec, total, sh, nb, f, cfg)) return 0;
+    const uint64_t fz = fusable(ctx, f, nb, cfg) ? 1 : 0;
+    return 2000000 + ((sig(f.d1) * 4096 + sig(f.d2)) * 4096 + (f.hasL ? sig(f.dL) : 0)) * 2 + fz;
 }
 
 // Every twiddle table an FFT of any register of the context can ask for.

@@ -69,6 +69,9 @@ SIGNATURES = {
     "cvdvSynchronize": ([c_void_p], None),
     "cvdvFlush": ([c_void_p], None),
     "cvdvSetTransformFusion": ([c_void_p, c_int], None),
+    "cvdvFftAutotune": ([c_void_p], c_int),
+    "cvdvFftGetTuning": ([c_void_p, c_int, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_double)],
+                         c_int),
     "cvdvGetStateDevice": ([c_void_p, c_void_p], None),
     "cvdvProfileEnable": ([c_void_p, c_int], None),
     "cvdvProfileRead": ([c_void_p, c_int, POINTER(c_double), POINTER(c_double)], c_int),

@@ -314,6 +314,21 @@ class HipCvdv:
         """Apply deferred phases and restore the canonical register layout now."""
         self._c("cvdvFlush")
 
+    def autotune(self) -> dict:
+        """Time the FFT plans of every register (cvdvFftAutotune, an addition) and
+        keep the fastest; returns {register: {"row_elems", "tile_cap", "fuse",
+        "ms_per_transform"}} for the tuned registers (0 / -1: planner default)."""
+        n = self._lib.cvdvFftAutotune(self.ctx)
+        _lib.check()
+        out = {"plans_timed": n}
+        for r in range(self.num_registers):
+            w, cap, fuse, ms = c_int(0), c_int(0), c_int(0), c_double(0.0)
+            if self._lib.cvdvFftGetTuning(self.ctx, r, ctypes.byref(w), ctypes.byref(cap), ctypes.byref(fuse),
+                                          ctypes.byref(ms)) == 1:
+                out[r] = {"row_elems": w.value, "tile_cap": cap.value, "fuse": fuse.value,
+                          "ms_per_transform": ms.value}
+        return out
+
     def info(self) -> dict:
         return {"registers": self.qubit_counts.tolist(), "dims": self.register_dims,
                 "grid_steps": self.grid_steps.tolist(), "total_size": self.total_size,

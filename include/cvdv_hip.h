@@ -107,6 +107,23 @@ void   cvdvFlush(CVDVContext* ctx);
  * runs every transform whole (two passes each), the reference's pass structure;
  * results agree to FFT rounding.  Default on. */
 void   cvdvSetTransformFusion(CVDVContext* ctx, int on);
+/* FFT autotuner (the reference leaves plan choices to cuFFT, context.cu:48-65).
+ * For every register field of the context it times the distinct plans of the
+ * planner's free choices (columns per strided tile, how far a tile grows over
+ * row blocks, transform fusion on or off) as chains of forward and inverse
+ * transforms on a private context of the same shape (its own state buffers,
+ * allocated for the call: the caller's state is not touched), and keeps the
+ * fastest per field in this context (another plan replaces the default only
+ * when it is at least 2% faster).  Results are kept per (precision, state
+ * size, field) for the process: a later call for the same shape times nothing.
+ * Completes a pending half transform first.  Returns the number of plans
+ * timed, or -1 on error. */
+int    cvdvFftAutotune(CVDVContext* ctx);
+/* The choices in force for register regIdx's field: 1 and the tuned values
+ * (0 / -1 members: the planner's default) with the timed milliseconds per
+ * transform, 0 when the field was not tuned, -1 on error. */
+int    cvdvFftGetTuning(CVDVContext* ctx, int regIdx, int* rowElems, int* tileCap, int* fuse,
+                        double* msPerTransform);
 /* Copies the state (canonical layout, complex128) to a caller device buffer. */
 void   cvdvGetStateDevice(CVDVContext* ctx, void* d_dst);
 /* Per-launch timing with HIP events recorded on the context's stream.

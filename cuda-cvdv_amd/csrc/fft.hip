@@ -518,32 +518,6 @@ __device__ __forceinline__ C gen_factor(const PreLds& P, const FastDiag& pre, ui
     return unit_rev_fast(cvdv_frac(r), R(0));
 }
 
-// Every generic term is affine in the row index of a tile whose rows are the
-// bits [rowLo, rowLo + rowN) (column and row-block bits fixed): each field holds
-// all of the row bits or none (a bilinear term at most one field with them, a
-// polynomial with them only its linear coefficient), and no control or Z bit
-// is a row bit.  Uniform across the tile.
-__device__ __forceinline__ bool gen_affine_in_rows(const PreLds& P, const FastDiag& pre, int rowLo, int rowN) {
-    auto rel = [&](int sh, int nb) {  // 0: misses the rows, 1: holds them all, 2: part of them
-        if (!(sh < rowLo + rowN && rowLo < sh + nb)) return 0;
-        return (sh <= rowLo && rowLo + rowN <= sh + nb) ? 1 : 2;
-    };
-    for (int i = 0; i < pre.ngen; ++i) {
-        const DiagTerm& T = P.gen[i];
-        if (T.ctrl >= rowLo && T.ctrl < rowLo + rowN) return false;
-        if (T.kind == DT_POLY) {
-            const int r = rel(T.sh1, T.nb1);
-            if (r == 2 || (r == 1 && (T.c2 != 0.0 || T.c3 != 0.0))) return false;
-        } else if (T.kind == DT_BILIN) {
-            const int r1 = rel(T.sh1, T.nb1), r2 = rel(T.sh2, T.nb2);
-            if (r1 == 2 || r2 == 2 || (r1 == 1 && r2 == 1)) return false;
-        } else if (T.sh1 >= rowLo && T.sh1 < rowLo + rowN) {
-            return false;
-        }
-    }
-    return true;
-}
-
 // Stage-0 inputs straight from global memory (issued early: a tile ahead).
 template <int RP, int E, int LGM, int LGC, typename C>
 __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, const FftPass& d, uint64_t inBase,
@@ -572,7 +546,8 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
 //             and is staged by row; generic terms are evaluated one element per
 //             iteration of a rolled loop into the free exchange buffer, which
 //             keeps their FP64 temporaries off the other elements' registers.
-enum { TM_ROWS = 0, TM_ANY = 1, TM_STAGED = 2 };
+// TM_NONE: the pass applies no phases (only the four-step twiddle, if TW4).
+enum { TM_ROWS = 0, TM_ANY = 1, TM_STAGED = 2, TM_NONE = 3 };
 template <int RP, int E, int LGM, int LGC, bool TW4, int TM, typename C>
 __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBase, const PreLds& P,
                                            const FastDiag& pre, const C* __restrict__ tw4, int tid, int nt,
@@ -622,7 +597,7 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             }
         }
     }
-    if (d.usePre) {
+    if (TM != TM_NONE && d.usePre) {
         int tt = tid;
         auto elem = [&](int k, int i) {
             const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
@@ -761,34 +736,35 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
             if (pre.ngen > 0) {
                 // One element per iteration: its factor goes to its own slot of the
                 // exchange buffer (element-major, so a wave's writes are
-                // consecutive), then every element takes its product.  complex64,
-                // every generic term affine in the row index: a butterfly's rows are
-                // q + i M/RP (column and row block fixed), so its phases are P0 + i D
-                // from two exact FP64 evaluations per butterfly.
-                const bool aff = sizeof(C) == 8 && gen_affine_in_rows(P, pre, d.lgInR, LGM);
+                // consecutive), then every element takes its product.  complex64:
+                // every generic term is affine in the row index (the host checked,
+                // gen_affine_in_rows()), and a butterfly's rows are q + i M/RP
+                // (column and row block fixed), so its phases are P0 + i D from
+                // two exact FP64 evaluations per butterfly.  complex128: each
+                // element's terms are evaluated exactly.
 #pragma unroll 1
                 for (int k = 0; k < NB; ++k) {
-                    double p0 = pre.genrev, dl = 0.0;
-                    if (aff) {
+                    if constexpr (sizeof(C) == 8) {
                         const uint64_t g0 = elem(k, 0), g1 = elem(k, 1);
+                        double p0 = pre.genrev, dl = 0.0;
                         for (int t = 0; t < pre.ngen; ++t) {
                             const double v0 = diag_term_rev(P.gen[t], g0);
                             p0 += cvdv_frac(v0);
                             dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - v0);
                         }
                         dl = cvdv_frac(dl);
-                    }
 #pragma unroll 1
-                    for (int i = 0; i < RP; ++i) {
-                        double r;
-                        if (aff) {
-                            r = fma((double)i, dl, p0);
-                        } else {
+                        for (int i = 0; i < RP; ++i)
+                            sRow[(k * RP + i) * nt + tt] =
+                                unit_rev_fast(cvdv_frac(fma((double)i, dl, p0)), decltype(C{}.x)(0));
+                    } else {
+#pragma unroll 1
+                        for (int i = 0; i < RP; ++i) {
                             const uint64_t g = elem(k, i);
-                            r = pre.genrev;
+                            double r = pre.genrev;
                             for (int t = 0; t < pre.ngen; ++t) r += cvdv_frac(diag_term_rev(P.gen[t], g));
+                            sRow[(k * RP + i) * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
                         }
-                        sRow[(k * RP + i) * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
                     }
                 }
 #pragma unroll
@@ -1025,8 +1001,12 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
         }
         prephase(x, t, tid);
-        stages(x, tid, twl);
-        store(x, t, tid);
+        // an opaque copy of the thread id for the stages and the store: their
+        // index math would otherwise be computed early and held across the phases
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        stages(x, t2, twl);
+        store(x, t, t2);
     } else {
         // Persistent workgroups (tile t = blockIdx.x, + gridDim.x, ...): the next
         // tile's inputs are loaded into registers before the current tile is
@@ -1087,7 +1067,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
 // the elements go through LDS from the last stage's order to stage 0's.
 template <typename R, int LGM, int LGC, int E, bool STG>
 __global__ void __launch_bounds__((tile_threads_max<R, LGM, LGC>()))
-__attribute__((amdgpu_waves_per_eu((STG && sizeof(R) == 4) ? 6 : 1)))
+__attribute__((amdgpu_waves_per_eu((STG && sizeof(R) == 4) ? 8 : 1)))
 k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int conjMid, int swapSh, int swapN,
             const typename CT<R>::type* __restrict__ tw, const typename CT<R>::type* __restrict__ tw4) {
     using C = typename CT<R>::type;
@@ -1140,7 +1120,7 @@ k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int
         for (int i = 0; i < E; ++i) x[i].y = -x[i].y;
     }
     // pending half: the four-step twiddle (d carries no phases and no input conjugation)
-    stage0_pre<R0, E, LGM, LGC, true, TM_ROWS>(x, d, base, P, pre, tw4, tid, nt, lds);
+    stage0_pre<R0, E, LGM, LGC, true, TM_NONE>(x, d, base, P, pre, tw4, tid, nt, lds);
     stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
     run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
     if (conjMid) {
@@ -1152,12 +1132,17 @@ k_fft_fused(typename CT<R>::type* data, FftPass d, FftPass dD, FastDiag pre, int
     lds_barrier();
     stage_read<R0, E, LGM, LGC>(lds, x, tid, nt);
     lds_barrier();
-    // phases at the natural index, then the next transform's first half
-    stage0_pre<R0, E, LGM, LGC, false, STG ? TM_STAGED : TM_ANY>(x, dD, natBase, P, pre, tw4, tid, nt, lds);
-    stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, tid, nt);
-    run_stages<LGM, LGC, 1, E, false>(x, lds, tw, tid, nt);
+    // phases at the natural index, then the next transform's first half.  An
+    // opaque copy of the thread id: index math derived from it (the store
+    // addresses above all) would otherwise be computed early and held across
+    // the phases in registers that the workgroup size leaves no room for.
+    int t2 = tid;
+    asm volatile("" : "+v"(t2));
+    stage0_pre<R0, E, LGM, LGC, false, STG ? TM_STAGED : TM_ANY>(x, dD, natBase, P, pre, tw4, t2, nt, lds);
+    stage_compute<R0, E, LGM, LGC, 0, false>(x, tw, t2, nt);
+    run_stages<LGM, LGC, 1, E, false>(x, lds, tw, t2, nt);
     out_finish<E>(x, d, scale, scaleLo);
-    last_store<RL, E, LGM, LGC>(data, x, d, base, scale, scaleLo, tid, nt);
+    last_store<RL, E, LGM, LGC>(data, x, d, base, scale, scaleLo, t2, nt);
 }
 
 // Direct DFT for very small states (fewer than 8 elements per tile).
@@ -1540,6 +1525,32 @@ static bool tile_tables_by_row(const FastDiag& pre, const FftPass& d) {
     return true;
 }
 
+// Every generic term is affine in the row index of a tile whose rows are the
+// bits [rowLo, rowLo + rowN) (column and row-block bits fixed): each field holds
+// all of the row bits or none (a bilinear term at most one field with them, a
+// polynomial with them only its linear coefficient), and no control or Z bit
+// is a row bit.
+static bool gen_affine_in_rows(const FastDiag& pre, int rowLo, int rowN) {
+    auto rel = [&](int sh, int nb) {  // 0: misses the rows, 1: holds them all, 2: part of them
+        if (!(sh < rowLo + rowN && rowLo < sh + nb)) return 0;
+        return (sh <= rowLo && rowLo + rowN <= sh + nb) ? 1 : 2;
+    };
+    for (int i = 0; i < pre.ngen; ++i) {
+        const DiagTerm& T = pre.gen[i];
+        if (T.ctrl >= rowLo && T.ctrl < rowLo + rowN) return false;
+        if (T.kind == DT_POLY) {
+            const int r = rel(T.sh1, T.nb1);
+            if (r == 2 || (r == 1 && (T.c2 != 0.0 || T.c3 != 0.0))) return false;
+        } else if (T.kind == DT_BILIN) {
+            const int r1 = rel(T.sh1, T.nb1), r2 = rel(T.sh2, T.nb2);
+            if (r1 == 2 || r2 == 2 || (r1 == 1 && r2 == 1)) return false;
+        } else if (T.sh1 >= rowLo && T.sh1 < rowLo + rowN) {
+            return false;
+        }
+    }
+    return true;
+}
+
 template <typename R, int LGM, int LGC>
 static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, const FftPass& d, const FftPass& dD,
                               const FastDiag& pre, int conjMid, int swapSh, int swapN,
@@ -1548,11 +1559,13 @@ static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, cons
     if constexpr (tile_instance<R, LGM, LGC>()) {
         if (tile / 8 > tile_threads_max<R, LGM, LGC>())
             throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile larger than its kernel's workgroup bound"};
-        // tiles of several columns whose tables are all read by row take the
-        // instance that stages them (and the generic terms) through LDS: fewer
-        // registers, more workgroups per CU
+        // tiles of several columns whose tables are all read by row (and, in
+        // complex64, whose generic terms are affine in the row index) take the
+        // instance that stages them through LDS: fewer registers, more
+        // workgroups per CU
         if constexpr (LGC > 0) {
-            if (!dD.usePre || tile_tables_by_row(pre, dD)) {
+            const bool gen = sizeof(R) == 8 || pre.ngen == 0 || gen_affine_in_rows(pre, dD.lgInR, LGM);
+            if (!dD.usePre || (tile_tables_by_row(pre, dD) && gen)) {
                 hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8, true>), dim3((unsigned)blocks), dim3(tile / 8), lds,
                                    ctx->stream, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
                 return;

@@ -88,6 +88,7 @@ struct BatchArgs {
     void* out;
     void* work;
     int twOff[BATCH_MAX_REG_QUBITS + 1];  // stage-table offset for FFT length 2^nb
+    uint32_t ldsCap;  // elements of dynamic LDS per workgroup (staging for states in the work area)
 };
 
 // ---------------------------------------------------------------- device helpers
@@ -192,37 +193,68 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
 // output scale.  Rows are processed in chunks of at most BATCH_EMAX * nt
 // elements; rows are independent, so chunks need no synchronisation between
 // them beyond the per-stage barriers.
+// The Stockham stages of one chunk of rows (radix 8, then one radix 4 or 2).
+template <typename C, typename S>
+__device__ __forceinline__ void bstages(S st, int sh, int nb, int lgC, uint32_t rho0, const C* tw, const BOp& op,
+                                        float schi, float sclo, const SProg& P, bool usePre, int tid, int nt) {
+    int rem = nb, lgL = 0, off = 0;
+    while (rem > 0) {
+        const int r = rem >= 3 ? 3 : rem;
+        const bool first = lgL == 0, last = rem == r;
+        if (r == 3)
+            bstage<8>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                      usePre, tid, nt);
+        else if (r == 2)
+            bstage<4>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                      usePre, tid, nt);
+        else
+            bstage<2>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                      usePre, tid, nt);
+        off += (1 << lgL) * ((1 << r) - 1);
+        lgL += r;
+        rem -= r;
+    }
+}
+
+// A state in the global work area (L2-resident) takes each chunk of rows
+// through the workgroup's LDS when it has room (stage): copied in with the
+// load program applied at each element's flat index, transformed there, copied
+// back; one L2 round trip per chunk instead of one per Stockham stage.
 template <typename C, typename S>
 __device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, const SProg& P, int tid,
-                                     int nt, int lgEN) {
+                                     int nt, int lgEN, C* stage, uint32_t stageCap) {
     const int nb = op.nb, sh = op.sh;
     const int lgRows = T - nb;
     const int lgC = min(lgRows, lgEN - nb);
     const uint32_t nChunks = 1u << (lgRows - lgC);
-    // the load program is applied by the first stage as it loads each element
     const bool usePre = P.active();
     const float schi = (float)op.outScale;
     const float sclo = (float)(op.outScale - (double)schi);
-    for (uint32_t ch = 0; ch < nChunks; ++ch) {
-        const uint32_t rho0 = ch << lgC;
-        int rem = nb, lgL = 0, off = 0;
-        while (rem > 0) {
-            const int r = rem >= 3 ? 3 : rem;
-            const bool first = lgL == 0, last = rem == r;
-            if (r == 3)
-                bstage<8>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          P, usePre, tid, nt);
-            else if (r == 2)
-                bstage<4>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          P, usePre, tid, nt);
-            else
-                bstage<2>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo,
-                          P, usePre, tid, nt);
-            off += (1 << lgL) * ((1 << r) - 1);
-            lgL += r;
-            rem -= r;
+    const uint32_t chunk = 1u << (nb + lgC);
+    if (stage && chunk <= stageCap) {
+        const uint32_t lomask = (1u << sh) - 1u, jmask = (1u << nb) - 1u;
+        for (uint32_t ch = 0; ch < nChunks; ++ch) {
+            const uint32_t rho0 = ch << lgC;
+            for (uint32_t e = tid; e < chunk; e += nt) {
+                const uint32_t rho = rho0 + (e >> nb);
+                const uint32_t g = ((rho >> sh) << (sh + nb)) | (rho & lomask) | ((e & jmask) << sh);
+                C v = st[g];
+                if (usePre) v = prog_apply(P, g, v);
+                stage[e] = v;
+            }
+            __syncthreads();
+            bstages<C>(stage, 0, nb, lgC, 0u, tw, op, schi, sclo, P, false, tid, nt);
+            for (uint32_t e = tid; e < chunk; e += nt) {
+                const uint32_t rho = rho0 + (e >> nb);
+                st[((rho >> sh) << (sh + nb)) | (rho & lomask) | ((e & jmask) << sh)] = stage[e];
+            }
+            __syncthreads();  // the next chunk overwrites the staging buffer
         }
+        return;
     }
+    // the load program is applied by the first stage as it loads each element
+    for (uint32_t ch = 0; ch < nChunks; ++ch)
+        bstages<C>(st, sh, nb, lgC, ch << lgC, tw, op, schi, sclo, P, usePre, tid, nt);
 }
 
 // Qubit gate and conditional parity: the same pair maps and FP64 matrix
@@ -289,7 +321,7 @@ __device__ __forceinline__ SProg load_prog(DiagTerm* sT, BGroup* sG, const Batch
 
 template <typename C, typename S>
 __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchArgs& a, DiagTerm* sT, BGroup* sG,
-                                            int tid, int nt, int lgEN) {
+                                            int tid, int nt, int lgEN, C* stage) {
     const uint32_t size = 1u << cd.T;
     // 1. product state in the canonical layout (k_init_product order: register 0 first)
     {
@@ -314,7 +346,7 @@ __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchAr
         if (op.tn > 0 || op.gn > 0) __syncthreads();
         switch (op.kind) {
             case BOP_FFT:
-                bfft<C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN);
+                bfft<C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN, stage, a.ldsCap);
                 break;
             case BOP_QUBIT: bqubit<C>(st, size, op, P, tid, nt); break;
             case BOP_CPARITY: bcparity<C>(st, size, op, P, tid, nt); break;
@@ -349,9 +381,9 @@ __global__ __launch_bounds__(BATCH_MAX_THREADS) void k_batch(BatchArgs a, int lg
     const BCirc& cd = a.circ[a.order[blockIdx.x]];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (cd.lds)
-        run_circuit<C>((C*)smem, cd, a, sT, sG, tid, nt, lgEN);
+        run_circuit<C>((C*)smem, cd, a, sT, sG, tid, nt, lgEN, (C*)nullptr);
     else
-        run_circuit<C>((C*)a.work + cd.workOff, cd, a, sT, sG, tid, nt, lgEN);
+        run_circuit<C>((C*)a.work + cd.workOff, cd, a, sT, sG, tid, nt, lgEN, (C*)smem);
 }
 
 // ---------------------------------------------------------------- readout kernel
@@ -680,9 +712,16 @@ static void batch_compile(CVDVBatch* b) {
     a.out = b->dOut;
     a.work = b->dWork;
     for (int i = 0; i <= BATCH_MAX_REG_QUBITS; ++i) a.twOff[i] = b->twOff[i];
-    bool anyLds = false;
-    for (const BCirc& d : hc) anyLds |= d.lds != 0;
-    const size_t shm = anyLds ? ldsElems * eb : 0;
+    bool anyLds = false, anyWork = false;
+    for (const BCirc& d : hc) {
+        anyLds |= d.lds != 0;
+        anyWork |= d.lds == 0;
+    }
+    // circuits in the work area stage their FFT chunks (at most EMAX * threads
+    // elements) in the same dynamic LDS
+    const size_t stageElems = (size_t)1 << lgEN;
+    const size_t shm = std::max(anyLds ? ldsElems * eb : 0, anyWork ? stageElems * eb : 0);
+    a.ldsCap = (uint32_t)(shm / eb);
     if (b->prec == CVDV_C128)
         HIPCHK(hipFuncSetAttribute((const void*)k_batch<double2>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    (int)std::max<size_t>(shm, 1)));

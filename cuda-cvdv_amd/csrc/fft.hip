@@ -1003,8 +1003,9 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
         prephase(x, t, tid);
         // an opaque copy of the thread id for the stages and the store: their
         // index math would otherwise be computed early and held across the phases
+        // (8-element tiles; the 16-element one-row tile has no register left for it)
         int t2 = tid;
-        asm volatile("" : "+v"(t2));
+        if constexpr (E == 8) asm volatile("" : "+v"(t2));
         stages(x, t2, twl);
         store(x, t, t2);
     } else {

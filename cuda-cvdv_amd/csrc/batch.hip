@@ -99,11 +99,11 @@ struct SProg {
     const BGroup* gr;
     int tn, gn;
     double grev;
-    __device__ bool active() const { return tn > 0 || gn > 0 || grev != 0.0; }
+    __device__ __forceinline__ bool active() const { return tn > 0 || gn > 0 || grev != 0.0; }
 };
 
 template <typename C>
-__device__ __forceinline__ C prog_apply(const SProg& P, uint32_t g, C v) {
+__device__ __forceinline__ C prog_apply(SProg P, uint32_t g, C v) {
     using R = decltype(v.x);
     double r = P.grev;
     for (int k = 0; k < P.gn; ++k) {
@@ -126,15 +126,39 @@ __device__ __forceinline__ float2 scale_by(float2 v, double, float hi, float lo)
     return make_float2(fmaf(v.x, hi, v.x * lo), fmaf(v.y, hi, v.y * lo));
 }
 
+// A state or staging buffer in LDS is stored with one pad slot after every 8
+// elements (index i at i + i/8).  The Stockham stages' first two radix-8
+// exchanges write at strides of 8 and 64 elements; padded, those become 9 and
+// 72, which fall on distinct 8-byte LDS slots across 32 lanes instead of 16-
+// and 8-way bank conflicts.  States in global memory are not padded.
+#define BATCH_LDS_PAD 3
+__host__ __device__ constexpr uint32_t lds_padded(uint32_t n) { return n + (n >> BATCH_LDS_PAD); }
+
+template <typename C, bool PAD>
+struct BState {
+    C* p;
+    __device__ __forceinline__ C& operator[](uint32_t i) const { return p[PAD ? i + (i >> BATCH_LDS_PAD) : i]; }
+};
+
+// Barrier between the phases of an op.  A state in LDS needs only LDS ordering;
+// a state in the global work area also needs this thread's stores complete.
+template <bool GST>
+__device__ __forceinline__ void bsync() {
+    if constexpr (GST) __syncthreads();
+    else lds_barrier();
+}
+
 // One in-place Stockham stage of radix R over a chunk of 2^lgC rows of the
 // register field [sh, sh+nb).  Row rho (0 <= rho < 2^(T-nb)) is the flat index
 // with the field removed; the chunk is rows rho0 .. rho0 + 2^lgC - 1.
 // Butterfly b of a row (sub-length L = 2^lgL done) reads rows b + t N/R and
-// writes (b / L) R L + (b mod L) + t L, t < R.
-template <int R, typename C, typename S>
+// writes (b / L) R L + (b mod L) + t L, t < R.  GST: the state is in global
+// memory (else in LDS).  This general form serves fields above the low bits
+// and short fields; fstage below covers the rest.
+template <int R, bool GST, typename C, typename S>
 __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t rho0, int lgL, const C* tw,
                                        bool first, bool last, bool inv, double sc, float schi, float sclo,
-                                       const SProg& P, bool usePre, int tid, int nt) {
+                                       SProg P, bool usePre, int tid, int nt) {
     constexpr int LGR = R == 8 ? 3 : (R == 4 ? 2 : 1);
     constexpr int NBF = BATCH_EMAX / R;
     const int lgQ = nb - LGR;
@@ -159,7 +183,7 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
             }
         }
     }
-    __syncthreads();
+    bsync<GST>();
 #pragma unroll
     for (int k = 0; k < NBF; ++k) {
         const uint32_t q = (uint32_t)tid + (uint32_t)k * nt;
@@ -186,7 +210,7 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
             }
         }
     }
-    __syncthreads();
+    bsync<GST>();
 }
 
 // Centred-DFT core along the field: load program, plain FFT (sign by inv),
@@ -194,35 +218,133 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
 // elements; rows are independent, so chunks need no synchronisation between
 // them beyond the per-stage barriers.
 // The Stockham stages of one chunk of rows (radix 8, then one radix 4 or 2).
-template <typename C, typename S>
+template <bool GST, typename C, typename S>
 __device__ __forceinline__ void bstages(S st, int sh, int nb, int lgC, uint32_t rho0, const C* tw, const BOp& op,
-                                        float schi, float sclo, const SProg& P, bool usePre, int tid, int nt) {
+                                        float schi, float sclo, SProg P, bool usePre, int tid, int nt) {
     int rem = nb, lgL = 0, off = 0;
     while (rem > 0) {
         const int r = rem >= 3 ? 3 : rem;
         const bool first = lgL == 0, last = rem == r;
         if (r == 3)
-            bstage<8>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
-                      usePre, tid, nt);
+            bstage<8, GST>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                           usePre, tid, nt);
         else if (r == 2)
-            bstage<4>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
-                      usePre, tid, nt);
+            bstage<4, GST>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                           usePre, tid, nt);
         else
-            bstage<2>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
-                      usePre, tid, nt);
+            bstage<2, GST>(st, sh, nb, lgC, rho0, lgL, tw + off, first, last, op.inv, op.outScale, schi, sclo, P,
+                           usePre, tid, nt);
         off += (1 << lgL) * ((1 << r) - 1);
         lgL += r;
         rem -= r;
     }
 }
 
+// The Stockham stages with the FFT length 2^NB fixed at compile time, for a
+// field in the low bits of an LDS-held state or staging buffer (padded layout:
+// row rho of the chunk starts at rho (N + N/8), element j of a row at j + j/8).
+// Lanes take consecutive butterflies of a row, so loads are contiguous; each
+// butterfly computes one padded base and reaches its R elements at
+// compile-time offsets (input stride N/R and output stride L are multiples of
+// 8, or L = 1 with R = 8 and an 8-aligned base).  Stage LGL of the plan: radix
+// 8 while at least 3 bits remain, then one radix 4 or 2; the twiddles of the
+// stages before it (all radix 8) take 2^LGL - 1 table entries.
+template <int NB, int LGL, typename C>
+__device__ __forceinline__ void fstage(C* s, int lgC, uint32_t rho0, const C* tw, bool inv, double sc, float schi,
+                                       float sclo, SProg P, bool usePre, int tid, int nt) {
+    constexpr int REM = NB - LGL;
+    constexpr int LGR = REM >= 3 ? 3 : REM;
+    constexpr int R = 1 << LGR;
+    constexpr bool FIRST = LGL == 0, LAST = REM == LGR;
+    constexpr int LGQ = NB - LGR, Q = 1 << LGQ, L = 1 << LGL;
+    constexpr uint32_t ROWP = (1u << NB) + (1u << NB) / 8;
+    constexpr int INS = Q + Q / 8;                  // padded input stride
+    constexpr int OUTS = L == 1 ? 1 : L + L / 8;   // padded output stride
+    static_assert(Q % 8 == 0 && (L == 1 ? R == 8 : L % 8 == 0), "padding must stay linear");
+    constexpr int NBF = BATCH_EMAX / R;
+    const uint32_t nbf = 1u << (lgC + LGQ);
+    // an opaque copy of the thread id per stage: the stages' addresses would
+    // otherwise all be computed up front and held across the whole chain
+    int ti = tid;
+    asm volatile("" : "+v"(ti));
+    C x[NBF][R];
+#pragma unroll
+    for (int k = 0; k < NBF; ++k) {
+        const uint32_t q = (uint32_t)ti + (uint32_t)k * nt;
+        if (q < nbf) {
+            const uint32_t rho = q >> LGQ, b = q & (Q - 1);
+            const C* src = s + rho * ROWP + b + (b >> 3);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                C v = src[t * INS];
+                if constexpr (FIRST) {
+                    if (usePre) v = prog_apply(P, ((rho0 + rho) << NB) | (b + (uint32_t)t * Q), v);
+                    if (inv) v.y = -v.y;  // inverse = conj . forward . conj
+                }
+                x[k][t] = v;
+            }
+        }
+    }
+    lds_barrier();
+#pragma unroll
+    for (int k = 0; k < NBF; ++k) {
+        const uint32_t q = (uint32_t)ti + (uint32_t)k * nt;
+        if (q < nbf) {
+            const uint32_t rho = q >> LGQ, b = q & (Q - 1);
+            const uint32_t p = b & (L - 1);
+            if constexpr (LGL > 0) {
+                const C* w = tw + ((1 << LGL) - 1) + p * (R - 1);
+#pragma unroll
+                for (int t = 1; t < R; ++t) x[k][t] = cmul(x[k][t], w[t - 1]);
+            }
+            dft_r<R, false>(x[k]);
+            const uint32_t d = ((b >> LGL) << (LGL + LGR)) + p;
+            C* dst = s + rho * ROWP + d + (d >> 3);
+#pragma unroll
+            for (int t = 0; t < R; ++t) {
+                C v = x[k][t];
+                if constexpr (LAST) {
+                    if (inv) v.y = -v.y;
+                    v = scale_by(v, sc, schi, sclo);
+                }
+                dst[t * OUTS] = v;
+            }
+        }
+    }
+    lds_barrier();
+    if constexpr (!LAST) fstage<NB, LGL + LGR, C>(s, lgC, rho0, tw, inv, sc, schi, sclo, P, usePre, tid, nt);
+}
+
+// Runs the fixed-length stages for 2^6 .. 2^12 points; false for other lengths.
+template <typename C>
+__device__ __forceinline__ bool fstages(int nb, C* s, int lgC, uint32_t rho0, const C* tw, const BOp& op,
+                                        float schi, float sclo, SProg P, bool usePre, int tid, int nt) {
+    switch (nb) {
+#define BATCH_FSTAGES_CASE(NB) \
+    case NB: fstage<NB, 0, C>(s, lgC, rho0, tw, op.inv, op.outScale, schi, sclo, P, usePre, tid, nt); return true;
+        BATCH_FSTAGES_CASE(6)
+        BATCH_FSTAGES_CASE(7)
+        BATCH_FSTAGES_CASE(8)
+        BATCH_FSTAGES_CASE(9)
+        BATCH_FSTAGES_CASE(10)
+        BATCH_FSTAGES_CASE(11)
+        BATCH_FSTAGES_CASE(12)
+#undef BATCH_FSTAGES_CASE
+        default: return false;
+    }
+}
+
 // A state in the global work area (L2-resident) takes each chunk of rows
 // through the workgroup's LDS when it has room (stage): copied in with the
 // load program applied at each element's flat index, transformed there, copied
-// back; one L2 round trip per chunk instead of one per Stockham stage.
-template <typename C, typename S>
-__device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, const SProg& P, int tid,
-                                     int nt, int lgEN, C* stage, uint32_t stageCap) {
+// back; one L2 round trip per chunk instead of one per Stockham stage.  A chunk
+// holds at most BATCH_EMAX elements per thread (the host sizes the launch so),
+// and each thread issues all its loads (stores) of a chunk before using them.
+// Chunks cover disjoint rows, so a chunk's stores may still be in flight while
+// the next chunk loads; the op's closing barrier completes them.
+template <bool GST, typename C, typename S>
+__device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, SProg P, int tid,
+                                     int nt, int lgEN, BState<C, true> stage, uint32_t stageCap) {
     const int nb = op.nb, sh = op.sh;
     const int lgRows = T - nb;
     const int lgC = min(lgRows, lgEN - nb);
@@ -231,36 +353,58 @@ __device__ __forceinline__ void bfft(S st, int T, const BOp& op, const C* tw, co
     const float schi = (float)op.outScale;
     const float sclo = (float)(op.outScale - (double)schi);
     const uint32_t chunk = 1u << (nb + lgC);
-    if (stage && chunk <= stageCap) {
+    if (stage.p && lds_padded(chunk) <= stageCap) {
         const uint32_t lomask = (1u << sh) - 1u, jmask = (1u << nb) - 1u;
+        auto flat = [&](uint32_t rho0, uint32_t e) {
+            const uint32_t rho = rho0 + (e >> nb);
+            return ((rho >> sh) << (sh + nb)) | (rho & lomask) | ((e & jmask) << sh);
+        };
         for (uint32_t ch = 0; ch < nChunks; ++ch) {
             const uint32_t rho0 = ch << lgC;
-            for (uint32_t e = tid; e < chunk; e += nt) {
-                const uint32_t rho = rho0 + (e >> nb);
-                const uint32_t g = ((rho >> sh) << (sh + nb)) | (rho & lomask) | ((e & jmask) << sh);
-                C v = st[g];
-                if (usePre) v = prog_apply(P, g, v);
-                stage[e] = v;
+            C v[BATCH_EMAX];
+#pragma unroll
+            for (int k = 0; k < BATCH_EMAX; ++k) {
+                const uint32_t e = (uint32_t)tid + (uint32_t)k * nt;
+                if (e < chunk) v[k] = st[flat(rho0, e)];
             }
-            __syncthreads();
-            bstages<C>(stage, 0, nb, lgC, 0u, tw, op, schi, sclo, P, false, tid, nt);
-            for (uint32_t e = tid; e < chunk; e += nt) {
-                const uint32_t rho = rho0 + (e >> nb);
-                st[((rho >> sh) << (sh + nb)) | (rho & lomask) | ((e & jmask) << sh)] = stage[e];
+#pragma unroll
+            for (int k = 0; k < BATCH_EMAX; ++k) {
+                const uint32_t e = (uint32_t)tid + (uint32_t)k * nt;
+                if (e < chunk) stage[e] = usePre ? prog_apply(P, flat(rho0, e), v[k]) : v[k];
             }
-            __syncthreads();  // the next chunk overwrites the staging buffer
+            lds_barrier();
+            if (!fstages<C>(nb, stage.p, lgC, 0u, tw, op, schi, sclo, P, false, tid, nt))
+                bstages<false, C>(stage, 0, nb, lgC, 0u, tw, op, schi, sclo, P, false, tid, nt);
+#pragma unroll
+            for (int k = 0; k < BATCH_EMAX; ++k) {
+                const uint32_t e = (uint32_t)tid + (uint32_t)k * nt;
+                if (e < chunk) v[k] = stage[e];
+            }
+#pragma unroll
+            for (int k = 0; k < BATCH_EMAX; ++k) {
+                const uint32_t e = (uint32_t)tid + (uint32_t)k * nt;
+                if (e < chunk) st[flat(rho0, e)] = v[k];
+            }
+            lds_barrier();  // the next chunk overwrites the staging buffer
         }
         return;
     }
     // the load program is applied by the first stage as it loads each element
-    for (uint32_t ch = 0; ch < nChunks; ++ch)
-        bstages<C>(st, sh, nb, lgC, ch << lgC, tw, op, schi, sclo, P, usePre, tid, nt);
+    for (uint32_t ch = 0; ch < nChunks; ++ch) {
+        const uint32_t rho0 = ch << lgC;
+        if constexpr (!GST) {
+            const uint32_t rowp = lds_padded(1u << nb);
+            if (sh == 0 && fstages<C>(nb, st.p + rho0 * rowp, lgC, rho0, tw, op, schi, sclo, P, usePre, tid, nt))
+                continue;
+        }
+        bstages<GST, C>(st, sh, nb, lgC, rho0, tw, op, schi, sclo, P, usePre, tid, nt);
+    }
 }
 
 // Qubit gate and conditional parity: the same pair maps and FP64 matrix
 // arithmetic as k_qubit / k_cparity (gates.hip); each pair belongs to one thread.
 template <typename C, typename S>
-__device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, const SProg& P, int tid, int nt) {
+__device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, SProg P, int tid, int nt) {
     const uint32_t lowmask = (1u << op.bit) - 1u;
     const bool usePre = P.active();
     for (uint32_t p = tid; p < size / 2; p += nt) {
@@ -281,7 +425,7 @@ __device__ __forceinline__ void bqubit(S st, uint32_t size, const BOp& op, const
 }
 
 template <typename C, typename S>
-__device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, const SProg& P, int tid, int nt) {
+__device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, SProg P, int tid, int nt) {
     const int top = op.sh + op.nb - 1;
     const uint32_t lowmask = (1u << top) - 1u;
     const uint32_t fmask = ((1u << op.nb) - 1u) << op.sh;
@@ -307,7 +451,7 @@ __device__ __forceinline__ void bcparity(S st, uint32_t size, const BOp& op, con
 }
 
 template <typename C, typename S>
-__device__ __forceinline__ void bdiag(S st, uint32_t size, const SProg& P, int tid, int nt) {
+__device__ __forceinline__ void bdiag(S st, uint32_t size, SProg P, int tid, int nt) {
     for (uint32_t g = tid; g < size; g += nt) st[g] = prog_apply(P, g, st[g]);
 }
 
@@ -319,44 +463,39 @@ __device__ __forceinline__ SProg load_prog(DiagTerm* sT, BGroup* sG, const Batch
     return SProg{sT, sG, tn, gn, grev};
 }
 
-template <typename C, typename S>
+template <bool GST, typename C, typename S>
 __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchArgs& a, DiagTerm* sT, BGroup* sG,
-                                            int tid, int nt, int lgEN, C* stage) {
+                                            int tid, int nt, int lgEN, BState<C, true> stage) {
     const uint32_t size = 1u << cd.T;
     // 1. product state in the canonical layout (k_init_product order: register 0 first)
-    {
-        uint32_t roff[BATCH_MAX_REG];
-        uint32_t o = cd.initOff;
-        for (int r = 0; r < cd.nreg; ++r) {
-            roff[r] = o;
-            o += 1u << cd.nq[r];
+    for (uint32_t g = tid; g < size; g += nt) {
+        uint32_t o = cd.initOff;  // register r's array follows register r-1's
+        double2 p = a.init[o + ((g >> cd.flw[0]) & ((1u << cd.nq[0]) - 1u))];
+        for (int r = 1; r < cd.nreg; ++r) {
+            o += 1u << cd.nq[r - 1];
+            p = cmul(p, a.init[o + ((g >> cd.flw[r]) & ((1u << cd.nq[r]) - 1u))]);
         }
-        for (uint32_t g = tid; g < size; g += nt) {
-            double2 p = a.init[roff[0] + ((g >> cd.flw[0]) & ((1u << cd.nq[0]) - 1u))];
-            for (int r = 1; r < cd.nreg; ++r)
-                p = cmul(p, a.init[roff[r] + ((g >> cd.flw[r]) & ((1u << cd.nq[r]) - 1u))]);
-            st[g] = cmk<C>(p.x, p.y);
-        }
+        st[g] = cmk<C>(p.x, p.y);
     }
-    __syncthreads();
+    bsync<GST>();
     // 2. ops
     for (uint32_t i = cd.opBegin; i < cd.opEnd; ++i) {
         const BOp& op = a.ops[i];
         const SProg P = load_prog(sT, sG, a, op.tfirst, op.tn, op.gfirst, op.gn, op.grev, tid);
-        if (op.tn > 0 || op.gn > 0) __syncthreads();
+        if (op.tn > 0 || op.gn > 0) lds_barrier();
         switch (op.kind) {
             case BOP_FFT:
-                bfft<C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN, stage, a.ldsCap);
+                bfft<GST, C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN, stage, a.ldsCap);
                 break;
             case BOP_QUBIT: bqubit<C>(st, size, op, P, tid, nt); break;
             case BOP_CPARITY: bcparity<C>(st, size, op, P, tid, nt); break;
             default: bdiag<C>(st, size, P, tid, nt); break;
         }
-        __syncthreads();
+        bsync<GST>();
     }
     // 3. canonical store with the trailing program
     const SProg F = load_prog(sT, sG, a, cd.ftFirst, cd.ftN, cd.fgFirst, cd.fgN, cd.fgrev, tid);
-    if (cd.ftN > 0 || cd.fgN > 0) __syncthreads();
+    if (cd.ftN > 0 || cd.fgN > 0) lds_barrier();
     const bool useFinal = F.active();
     C* out = (C*)a.out + cd.outOff;
     for (uint32_t g = tid; g < size; g += nt) {
@@ -381,9 +520,10 @@ __global__ __launch_bounds__(BATCH_MAX_THREADS) void k_batch(BatchArgs a, int lg
     const BCirc& cd = a.circ[a.order[blockIdx.x]];
     const int tid = threadIdx.x, nt = blockDim.x;
     if (cd.lds)
-        run_circuit<C>((C*)smem, cd, a, sT, sG, tid, nt, lgEN, (C*)nullptr);
+        run_circuit<false, C>(BState<C, true>{(C*)smem}, cd, a, sT, sG, tid, nt, lgEN, BState<C, true>{nullptr});
     else
-        run_circuit<C>((C*)a.work + cd.workOff, cd, a, sT, sG, tid, nt, lgEN, (C*)smem);
+        run_circuit<true, C>(BState<C, false>{(C*)a.work + cd.workOff}, cd, a, sT, sG, tid, nt, lgEN,
+                             BState<C, true>{(C*)smem});
 }
 
 // ---------------------------------------------------------------- readout kernel
@@ -631,7 +771,7 @@ static void batch_compile(CVDVBatch* b) {
             initOff += 1u << c->nq[r];
         }
         const size_t size = c->totalSize;
-        d.lds = size <= ldsElems ? 1 : 0;
+        d.lds = lds_padded((uint32_t)size) <= ldsElems ? 1 : 0;  // padded LDS layout
         if (!d.lds) {
             d.workOff = workElems;
             workElems += size;
@@ -718,8 +858,8 @@ static void batch_compile(CVDVBatch* b) {
         anyWork |= d.lds == 0;
     }
     // circuits in the work area stage their FFT chunks (at most EMAX * threads
-    // elements) in the same dynamic LDS
-    const size_t stageElems = (size_t)1 << lgEN;
+    // elements, padded) in the same dynamic LDS
+    const size_t stageElems = lds_padded(1u << lgEN);
     const size_t shm = std::max(anyLds ? ldsElems * eb : 0, anyWork ? stageElems * eb : 0);
     a.ldsCap = (uint32_t)(shm / eb);
     if (b->prec == CVDV_C128)
@@ -962,7 +1102,9 @@ double cvdvBatchAutotune(CVDVBatch* b, int reps) {
     if (!b->initSet) throw CvdvError{CVDV_ERR_STATE, "cvdvBatchAutotune: call cvdvBatchInitFromSeparable first"};
     if (reps < 1) reps = 1;
     const int threadsOpts[2] = {256, 512};
-    const int ldsOpts[5] = {0, 16384, 32768, 65536, 131072};
+    // budgets that hold padded states of 2^11 .. 2^14 complex64 (2^10 .. 2^13 complex128)
+    const int ldsOpts[5] = {0, (int)lds_padded(16384), (int)lds_padded(32768), (int)lds_padded(65536),
+                            (int)lds_padded(131072)};
     hipEvent_t e0, e1;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));

@@ -300,16 +300,6 @@ struct FftPass {
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
-// Workgroup barrier that orders LDS traffic only.  __syncthreads() also waits
-// for every outstanding global access (one counter covers loads and stores on
-// this architecture), which would stall the next tile's prefetched loads at
-// every Stockham exchange.
-__device__ __forceinline__ void lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
 // Padded index of i0 + m * STRIDE.  When STRIDE is a multiple of 32 the padding
 // is linear (adding it leaves the low five bits of i0 alone), so every element
 // is the padded base plus a compile-time offset (an immediate in the LDS access).

@@ -5,6 +5,16 @@
 
 #include "cvdv_internal.h"
 
+// Workgroup barrier that orders LDS traffic only.  __syncthreads() also waits
+// for every outstanding global access (one counter covers loads and stores on
+// this architecture), which would stall prefetched loads or stores still in
+// flight at every Stockham exchange.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 template <bool INV, typename C>
 __device__ __forceinline__ C mul_w(C a, C w) {
     return INV ? cmulc(a, w) : cmul(a, w);

@@ -206,7 +206,9 @@ size_t cvdvBatchStateSize(CVDVBatch* batch, int i);
 size_t cvdvBatchNumOps(CVDVBatch* batch);
 void   cvdvBatchInitFromSeparable(CVDVBatch* batch, const double* hostC128);
 /* Launch shape: threads per workgroup (256 or 512) and the LDS budget in
- * bytes; circuits whose state exceeds the budget run from a global work area.
+ * bytes; a state held in LDS takes 9/8 of its size (one pad slot per 8
+ * amplitudes), and circuits whose padded state exceeds the budget run from a
+ * global work area.
  * Autotune times every shape (reps runs each), keeps the fastest and returns
  * its time per run in microseconds. */
 void   cvdvBatchSetLaunch(CVDVBatch* batch, int threads, int ldsBytes);

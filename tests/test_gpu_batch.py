@@ -157,7 +157,7 @@ def test_batch_launch_shapes_bitwise(prec):
     print(f"[batch] launch shapes {prec}", flush=True)
     b, circuits = _make(prec, seed=3, n_gates=25)
     ref = None
-    for threads, lds in [(256, 0), (256, 16384), (512, 32768), (256, 131072), (512, 65536)]:
+    for threads, lds in [(256, 0), (256, 16384), (512, 32768), (256, 131072), (512, 65536), (512, 147456)]:
         print(f"[batch]   threads={threads} lds={lds}", flush=True)
         b.setLaunch(threads, lds)
         b.run()

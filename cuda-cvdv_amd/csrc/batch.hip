@@ -93,28 +93,61 @@ struct BatchArgs {
 
 // ---------------------------------------------------------------- device helpers
 // An op's load program as staged in LDS: merged polynomial groups, the other
-// (generic) terms, the global phase.
+// (generic) terms, the global phase.  tab (optional): the whole program
+// tabulated in LDS, entry (s << tnb) | j for the control pattern s (bit k:
+// control bit (tctl >> 8k) & 255) and the field index j (bits [tsh, tsh +
+// tnb)); each
+// entry is the factor the evaluation below computes for that (s, j).
 struct SProg {
     const DiagTerm* t;
     const BGroup* gr;
     int tn, gn;
     double grev;
+    const void* tab;
+    int tsh, tnb, tnctl, tctl;
     __device__ __forceinline__ bool active() const { return tn > 0 || gn > 0 || grev != 0.0; }
 };
+
+// Revolutions of group G at control pattern s and coordinate m, reduced mod 1.
+// Contraction off: the product is rounded on its own before the reduction, so
+// the per-element evaluation and a table entry (tabulate) agree bit for bit
+// wherever the compiler inlines them.
+__device__ __forceinline__ double group_rev(const BGroup& G, int s, double m) {
+#pragma clang fp contract(off)
+    const double y = fma(fma(G.c[s][2], m, G.c[s][1]), m, G.c[s][0]) * m;
+    return y - rint(y);
+}
+
+// a * b with a fixed choice of fused products (contraction off), the same
+// wherever it is inlined: a factor from a table and one computed in place give
+// the same product bit for bit.
+template <typename C>
+__device__ __forceinline__ C cmul_fixed(C a, C b) {
+#pragma clang fp contract(off)
+    C r;
+    r.x = fma(a.x, b.x, -(a.y * b.y));
+    r.y = fma(a.x, b.y, a.y * b.x);
+    return r;
+}
 
 template <typename C>
 __device__ __forceinline__ C prog_apply(SProg P, uint32_t g, C v) {
     using R = decltype(v.x);
+    if (P.tab) {
+        uint32_t s = 0;
+        for (int k = 0; k < P.tnctl; ++k) s |= ((g >> ((P.tctl >> (8 * k)) & 255)) & 1u) << k;
+        return cmul_fixed(v, ((const C*)P.tab)[(s << P.tnb) | ((g >> P.tsh) & ((1u << P.tnb) - 1u))]);
+    }
     double r = P.grev;
     for (int k = 0; k < P.gn; ++k) {
         const BGroup& G = P.gr[k];
         int s = 0;
         for (int j = 0; j < G.nctl; ++j) s |= (int)((g >> G.ctl[j]) & 1u) << j;
         const double m = (double)((int)(2u * ((g >> G.sh) & ((1u << G.nb) - 1u))) + 1 - (1 << G.nb));
-        r += cvdv_frac(fma(fma(G.c[s][2], m, G.c[s][1]), m, G.c[s][0]) * m);
+        r += group_rev(G, s, m);
     }
     for (int i = 0; i < P.tn; ++i) r += cvdv_frac(diag_term_rev(P.t[i], g));
-    return cmul(v, unit_rev_fast(cvdv_frac(r), R(0)));
+    return cmul_fixed(v, unit_rev_fast(cvdv_frac(r), R(0)));
 }
 
 __device__ __forceinline__ double2 scale_by(double2 v, double s, float, float) {
@@ -460,12 +493,51 @@ __device__ __forceinline__ SProg load_prog(DiagTerm* sT, BGroup* sG, const Batch
                                            int gfirst, int gn, double grev, int tid) {
     if (tid < tn) sT[tid] = a.terms[tfirst + tid];
     if (tid >= CVDV_MAX_TERMS && tid < CVDV_MAX_TERMS + gn) sG[tid - CVDV_MAX_TERMS] = a.groups[gfirst + tid - CVDV_MAX_TERMS];
-    return SProg{sT, sG, tn, gn, grev};
+    SProg P{};
+    P.t = sT;
+    P.gr = sG;
+    P.tn = tn;
+    P.gn = gn;
+    P.grev = grev;
+    return P;
 }
 
+// A program of one polynomial group and nothing else, over a state of 2^T
+// amplitudes: its factor depends on the group's field index and control bits
+// only, so its 2^(nb + nctl) distinct factors are tabulated in the free LDS
+// (tab, cap entries) when that at least halves the evaluations.  The entries
+// are computed exactly as prog_apply evaluates them.  Call after the program
+// is staged (and its barrier); it ends with a barrier when it builds.
+template <typename C>
+__device__ __forceinline__ void tabulate(SProg& P, int T, C* tab, uint32_t cap, int tid, int nt) {
+    using R = decltype(C{}.x);
+    if (P.gn != 1 || P.tn != 0 || !tab) return;
+    const BGroup& G = P.gr[0];
+    const int nb = G.nb, nctl = G.nctl;
+    const uint32_t n = 1u << (nb + nctl);
+    if (nb + nctl + 1 > T || n > cap) return;
+    for (uint32_t e = tid; e < n; e += nt) {
+        const uint32_t sg = e >> nb, j = e & ((1u << nb) - 1u);
+        const double m = (double)((int)(2u * j) + 1 - (1 << nb));
+        double r = P.grev;
+        r += group_rev(G, (int)sg, m);
+        tab[e] = unit_rev_fast(cvdv_frac(r), R(0));
+    }
+    P.tab = tab;
+    P.tsh = G.sh;
+    P.tnb = nb;
+    P.tnctl = nctl;
+    P.tctl = 0;
+    for (int k = 0; k < nctl; ++k) P.tctl |= G.ctl[k] << (8 * k);
+    lds_barrier();
+}
+
+// tab, tabCap: LDS the circuit leaves free (after its state or staging buffer),
+// where an op's program may be tabulated.
 template <bool GST, typename C, typename S>
 __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchArgs& a, DiagTerm* sT, BGroup* sG,
-                                            int tid, int nt, int lgEN, BState<C, true> stage) {
+                                            int tid, int nt, int lgEN, BState<C, true> stage, C* tab,
+                                            uint32_t tabCap) {
     const uint32_t size = 1u << cd.T;
     // 1. product state in the canonical layout (k_init_product order: register 0 first)
     for (uint32_t g = tid; g < size; g += nt) {
@@ -481,8 +553,9 @@ __device__ __forceinline__ void run_circuit(S st, const BCirc& cd, const BatchAr
     // 2. ops
     for (uint32_t i = cd.opBegin; i < cd.opEnd; ++i) {
         const BOp& op = a.ops[i];
-        const SProg P = load_prog(sT, sG, a, op.tfirst, op.tn, op.gfirst, op.gn, op.grev, tid);
+        SProg P = load_prog(sT, sG, a, op.tfirst, op.tn, op.gfirst, op.gn, op.grev, tid);
         if (op.tn > 0 || op.gn > 0) lds_barrier();
+        tabulate<C>(P, cd.T, tab, tabCap, tid, nt);
         switch (op.kind) {
             case BOP_FFT:
                 bfft<GST, C>(st, cd.T, op, (const C*)a.tw + a.twOff[op.nb], P, tid, nt, lgEN, stage, a.ldsCap);
@@ -519,11 +592,17 @@ __global__ __launch_bounds__(BATCH_MAX_THREADS) void k_batch(BatchArgs a, int lg
     __shared__ BGroup sG[BATCH_MAX_GROUPS];
     const BCirc& cd = a.circ[a.order[blockIdx.x]];
     const int tid = threadIdx.x, nt = blockDim.x;
+    // the LDS after the circuit's state (or its largest staged chunk) holds
+    // tabulated programs
+    const uint32_t used = lds_padded(cd.lds ? (1u << cd.T) : (1u << lgEN));
+    C* tab = used < a.ldsCap ? (C*)smem + used : nullptr;
+    const uint32_t tabCap = used < a.ldsCap ? a.ldsCap - used : 0u;
     if (cd.lds)
-        run_circuit<false, C>(BState<C, true>{(C*)smem}, cd, a, sT, sG, tid, nt, lgEN, BState<C, true>{nullptr});
+        run_circuit<false, C>(BState<C, true>{(C*)smem}, cd, a, sT, sG, tid, nt, lgEN, BState<C, true>{nullptr},
+                              tab, tabCap);
     else
         run_circuit<true, C>(BState<C, false>{(C*)a.work + cd.workOff}, cd, a, sT, sG, tid, nt, lgEN,
-                             BState<C, true>{(C*)smem});
+                             BState<C, true>{(C*)smem}, tab, tabCap);
 }
 
 // ---------------------------------------------------------------- readout kernel

@@ -120,7 +120,8 @@ __device__ __forceinline__ double group_rev(const BGroup& G, int s, double m) {
 
 // a * b with a fixed choice of fused products (contraction off), the same
 // wherever it is inlined: a factor from a table and one computed in place give
-// the same product bit for bit.
+// the same product bit for bit, and so do the twiddle products of the two
+// Stockham stage forms (bstage, fstage), whichever a launch shape runs.
 template <typename C>
 __device__ __forceinline__ C cmul_fixed(C a, C b) {
 #pragma clang fp contract(off)
@@ -228,7 +229,7 @@ __device__ __forceinline__ void bstage(S st, int sh, int nb, int lgC, uint32_t r
             if (lgL > 0) {
                 const C* w = tw + p * (R - 1);
 #pragma unroll
-                for (int t = 1; t < R; ++t) x[k][t] = cmul(x[k][t], w[t - 1]);
+                for (int t = 1; t < R; ++t) x[k][t] = cmul_fixed(x[k][t], w[t - 1]);
             }
             dft_r<R, false>(x[k]);
             const uint32_t d = ((b >> lgL) << (lgL + LGR)) + p;
@@ -300,12 +301,22 @@ __device__ __forceinline__ void fstage(C* s, int lgC, uint32_t rho0, const C* tw
     // otherwise all be computed up front and held across the whole chain
     int ti = tid;
     asm volatile("" : "+v"(ti));
+    // complex64: the stage's twiddles (global, cache resident) are loaded with
+    // its inputs, before the barrier, so their latency is not exposed after it
+    // (complex128 has no registers left for them)
+    constexpr bool TWPRE = LGL > 0 && sizeof(C) == 8;
     C x[NBF][R];
+    C w[TWPRE ? NBF : 1][R];
 #pragma unroll
     for (int k = 0; k < NBF; ++k) {
         const uint32_t q = (uint32_t)ti + (uint32_t)k * nt;
         if (q < nbf) {
             const uint32_t rho = q >> LGQ, b = q & (Q - 1);
+            if constexpr (TWPRE) {
+                const C* wp = tw + ((1 << LGL) - 1) + (b & (L - 1)) * (R - 1);
+#pragma unroll
+                for (int t = 1; t < R; ++t) w[k][t] = wp[t - 1];
+            }
             const C* src = s + rho * ROWP + b + (b >> 3);
 #pragma unroll
             for (int t = 0; t < R; ++t) {
@@ -325,10 +336,13 @@ __device__ __forceinline__ void fstage(C* s, int lgC, uint32_t rho0, const C* tw
         if (q < nbf) {
             const uint32_t rho = q >> LGQ, b = q & (Q - 1);
             const uint32_t p = b & (L - 1);
-            if constexpr (LGL > 0) {
-                const C* w = tw + ((1 << LGL) - 1) + p * (R - 1);
+            if constexpr (TWPRE) {
 #pragma unroll
-                for (int t = 1; t < R; ++t) x[k][t] = cmul(x[k][t], w[t - 1]);
+                for (int t = 1; t < R; ++t) x[k][t] = cmul_fixed(x[k][t], w[k][t]);
+            } else if constexpr (LGL > 0) {
+                const C* wp = tw + ((1 << LGL) - 1) + p * (R - 1);
+#pragma unroll
+                for (int t = 1; t < R; ++t) x[k][t] = cmul_fixed(x[k][t], wp[t - 1]);
             }
             dft_r<R, false>(x[k]);
             const uint32_t d = ((b >> LGL) << (LGL + LGR)) + p;

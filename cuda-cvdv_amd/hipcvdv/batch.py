@@ -86,6 +86,9 @@ class HipCvdvBatch:
         self._c("cvdvBatchInitFromSeparable", flat.ctypes.data_as(POINTER(c_double)))
 
     def setLaunch(self, threads: int, ldsBytes: int) -> None:
+        """Threads per workgroup (256 or 512) and the LDS budget in bytes.  A state held
+        in LDS takes 9/8 of its size (one pad slot per 8 amplitudes); larger states run
+        from a global work area.  Results do not depend on the shape."""
         self._c("cvdvBatchSetLaunch", c_int(threads), c_int(ldsBytes))
 
     def getLaunch(self) -> tuple:

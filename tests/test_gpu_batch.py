@@ -132,6 +132,41 @@ def test_batch_matches_sequential(prec):
         assert abs(norms[i] - np.sum(np.abs(st) ** 2)) < 1e-12
 
 
+@pytest.mark.parametrize("prec", ["complex128", "mixed"])
+def test_batch_tabulated_programs_match_sequential(prec):
+    """The largest LDS budget: circuits of up to 2^15 amplitudes whose ops' phase
+    programs (one polynomial group with up to three control qubits) are tabulated
+    in the LDS left free beside the state or the staged FFT chunk; against the
+    same circuits run one by one."""
+    print(f"[batch] tabulated programs vs sequential {prec}", flush=True)
+    shapes = [[3, 12], [2, 12], [3, 10], [1, 12], [3, 11], [2, 9]]
+    rng = np.random.default_rng(21)
+    circuits = []
+    for s in shapes:
+        gates = []
+        for g in _gate_list(rng, s, 50):
+            gates.append(g)
+            if g[0] in ("d", "r", "s", "sheer"):  # controlled phases on several qubits
+                qb = int(rng.integers(0, s[0]))
+                gates.append(("cd", 1, 0, qb, complex(*(0.3 * rng.uniform(-1, 1, 2)))))
+        circuits.append((s, _init_arrays(rng, s), gates))
+    from hipcvdv import HipCvdvBatch
+
+    b = HipCvdvBatch([c[0] for c in circuits], precision=prec)
+    for i, (_, _, gates) in enumerate(circuits):
+        sim = b.circuit(i)
+        for g in gates:
+            getattr(sim, g[0])(*g[1:])
+    b.initFromSeparable([c[1] for c in circuits])
+    b.setLaunch(512, 147456)
+    b.run()
+    seq = _sequential(prec, circuits)
+    tol = 1e-11 if prec == "complex128" else 2e-5
+    for i, (shape, _, _) in enumerate(circuits):
+        err = np.max(np.abs(b.getState(i) - seq[i][0]))
+        assert err < tol, (i, shape, err)
+
+
 def test_batch_matches_oracle_c128():
     from oracle.cvdv_oracle import CvdvOracle
     from oracle.separable_np import SeparableStateNP

@@ -191,6 +191,13 @@ def run_c4(prec, circuits, n_seq, steps):
     for _ in range(steps):
         probs, norms = batched()
     el = (time.perf_counter() - t0) / steps
+    # the launch alone (every circuit, no readouts): best of 3
+    kernel_s = 1e9
+    for _ in range(3):
+        t0 = time.perf_counter()
+        batch.run()
+        batch.synchronize()
+        kernel_s = min(kernel_s, time.perf_counter() - t0)
     # the same circuits issued one by one: one context per shape, reused
     sims = {}
     seps = []
@@ -219,7 +226,7 @@ def run_c4(prec, circuits, n_seq, steps):
     passes = sum(c4_ft_count(g) for c in circuits for g in c[2])
     amps = sum(1 << (c[0][0] + c[0][1]) for c in circuits)
     del sims, seps
-    return {"batched_ms": el * 1e3, "circuits_per_s": n / el, "record_s": record_s,
+    return {"batched_ms": el * 1e3, "kernel_ms": kernel_s * 1e3, "circuits_per_s": n / el, "record_s": record_s,
             "launch": {"threads": threads, "lds_bytes": lds, "autotuned_us": tune_us},
             "num_ops": batch.numOps(), "fft_passes": passes, "amplitudes": amps,
             "seq_circuits": n_seq, "seq_ms_per_circuit": seq_el / n_seq * 1e3,
@@ -229,8 +236,57 @@ def run_c4(prec, circuits, n_seq, steps):
             "probs0": probs, "norms": norms}
 
 
+def c4_pmc(key):
+    """A figure of the batch kernel from the committed counter summary of the C4
+    workload (profiles/k_batch_c4_pmc.txt, separate rocprofv3 runs), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "k_batch_c4_pmc.txt")) as f:
+            for line in f:
+                if line.startswith(key + " ="):
+                    return float(line.split("=")[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def c4_roofline(prec, r, circuits):
+    """The batch kernel keeps each circuit's state on chip (LDS, or a work area in
+    L2 / MALL): its algorithmic HBM bytes per launch are the inputs (one array per
+    register, complex128) and the canonical outputs.  achieved = those bytes over
+    the launch time; traffic = bytes per launch beyond the L2 (HBM and MALL) from
+    counters, mostly the work areas' per-op reads and writes; valu_issue = VALU
+    issue cycles over the SIMDs' cycles (counters).  Neither is near its peak: the
+    kernel is latency-bound (two waves per SIMD, a barrier per Stockham stage)."""
+    s = 8 if prec == "mixed" else 16
+    out_b = sum((1 << (c[0][0] + c[0][1])) * s for c in circuits)
+    in_b = sum(((1 << c[0][0]) + (1 << c[0][1])) * 16 for c in circuits)
+    alg = out_b + in_b
+    ach = alg / (r["kernel_ms"] * 1e-3) / 1e9
+    tag = "c64" if prec == "mixed" else "c128"
+    traffic = c4_pmc(f"traffic_bytes_per_launch[{tag}]")
+    return {"bound": "latency", "algorithmic_hbm_bytes": alg, "achieved_hbm_gbps": ach, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_gbps": traffic / (r["kernel_ms"] * 1e-3) / 1e9 if traffic else None,
+            "valu_issue": c4_pmc(f"valu_issue_frac[{tag}]"), "kernel_ms": r["kernel_ms"]}
+
+
+def calibration_copy():
+    """A plain device copy of 2^26 complex64 amplitudes (512 MiB read and written):
+    the known-size kernel that scales the FETCH_SIZE / WRITE_SIZE counters in
+    counter runs of a section that has no copy of its own."""
+    import torch
+
+    a = torch.zeros(1 << 26, dtype=torch.complex64, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+    torch.cuda.empty_cache()
+
+
 def c4_section(n, n_seq, steps):
     circuits = make_c4(n)
+    calibration_copy()
     res = {}
     for prec in ("mixed", "complex128"):
         res[prec] = run_c4(prec, circuits, n_seq, steps)
@@ -240,6 +296,8 @@ def c4_section(n, n_seq, steps):
               f"{res[prec]['speedup_vs_sequential']:.1f}x", file=sys.stderr, flush=True)
     pm, pd = res["mixed"].pop("probs0"), res["complex128"].pop("probs0")
     nm, nd = res["mixed"].pop("norms"), res["complex128"].pop("norms")
+    for prec in ("mixed", "complex128"):
+        res[prec]["roofline"] = c4_roofline(prec, res[prec], circuits)
     m = res["mixed"]
     return {
         "workload": f"{n} circuits [1-3 qubits, 2^8-2^12 oscillator], 50-200 gates "

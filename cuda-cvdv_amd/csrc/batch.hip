@@ -95,9 +95,9 @@ struct BatchArgs {
 // An op's load program as staged in LDS: merged polynomial groups, the other
 // (generic) terms, the global phase.  tab (optional): the whole program
 // tabulated in LDS, entry (s << tnb) | j for the control pattern s (bit k:
-// control bit (tctl >> 8k) & 255) and the field index j (bits [tsh, tsh +
-// tnb)); each
-// entry is the factor the evaluation below computes for that (s, j).
+// control bit (tctl >> 8k) & 255) and the field index j (bits [tsh,
+// tsh + tnb)); each entry is the factor the evaluation below computes for
+// that (s, j).
 struct SProg {
     const DiagTerm* t;
     const BGroup* gr;

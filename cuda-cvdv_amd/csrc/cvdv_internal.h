@@ -44,7 +44,9 @@ enum CvdvErr {
 };
 
 // ---------------------------------------------------------------- diag program
-enum DiagKind { DT_POLY = 1, DT_BILIN = 2, DT_ZBIT = 3 };
+// DT_LINF: a linear term c1 m over one field that its consumer evaluates in
+// place instead of reading a table (see diag_fold_linear); evaluated as DT_POLY.
+enum DiagKind { DT_POLY = 1, DT_BILIN = 2, DT_ZBIT = 3, DT_LINF = 4 };
 
 struct DiagTerm {
     int kind;
@@ -71,7 +73,7 @@ __device__ __forceinline__ double diag_field_m(uint64_t g, int sh, int nb) {
 // One term's phase of element g in revolutions (not yet reduced).
 __device__ __forceinline__ double diag_term_rev(const DiagTerm& t, uint64_t g) {
     double v;
-    if (t.kind == DT_POLY) {
+    if (t.kind == DT_POLY || t.kind == DT_LINF) {
         const double m = diag_field_m(g, t.sh1, t.nb1);
         v = fma(fma(t.c3, m, t.c2), m, t.c1) * m;
     } else if (t.kind == DT_BILIN) {

@@ -118,6 +118,12 @@ __device__ __forceinline__ C apply_fast(const FastDiag& f, uint64_t g, C v) {
 FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double scale = 1.0,
                       bool* scaleFolded = nullptr, uint64_t tileBits = 0);
 
+// p with its linear polynomial terms over exactly the field [sh, sh + nb)
+// (control bit outside it) marked DT_LINF, as many as the generic-term slots
+// allow: a consumer that folds such terms per butterfly (the one-row FFT tile)
+// evaluates them in place instead of reading a table per element.
+DiagProg diag_fold_linear(const DiagProg& p, int sh, int nb);
+
 // compile_diag(p, ..., tileBits) launches nothing but the table build: no
 // FP64 pass and no table pass of its own (with a table slot to spare).
 bool diag_fits(const DiagProg& p, uint64_t tileBits);

@@ -482,7 +482,7 @@ __device__ __forceinline__ GenFold fold_generic(const PreLds& P, const FastDiag&
                 f.a += cvdv_frac(sg * T.c1 * diag_field_m(g0, T.sh1, T.nb1));
             else
                 f.b += cvdv_frac(sg * diag_term_rev(T, g0));
-        } else if (T.kind == DT_POLY) {
+        } else if (T.kind == DT_POLY || T.kind == DT_LINF) {
             if (in1 && T.c2 == 0.0 && T.c3 == 0.0)
                 f.a += cvdv_frac(sg * T.c1);
             else if (field_overlaps(T.sh1, T.nb1, fs, fn))
@@ -1422,8 +1422,8 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     const FftCfg cfg = field_cfg(ctx, total, sh, nb);
     // The pending phases are compiled once the tiling of the pass that applies
     // them is known: their tables take no choice bit that varies within a tile.
-    auto compile_pre = [&](uint64_t tileBits) {
-        const FastDiag f = compile_diag(ctx, preProg, act, outScale, &folded, tileBits);
+    auto compile_pre = [&](uint64_t tileBits, const DiagProg& prog) {
+        const FastDiag f = compile_diag(ctx, prog, act, outScale, &folded, tileBits);
         return f;
     };
 
@@ -1431,7 +1431,12 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     PassGeom g1{(int)N, S, H, 1, 0, N * S, S, 0, N * S, S};
     FftPass d{};
     if (fit_pass(g1, prec, d, true, cfg)) {
-        const FastDiag pre = compile_pre(tile_bits(d));
+        // A persistent complex64 one-row pass folds the linear terms over its row
+        // field per butterfly (one sin/cos each): a table would cost a global
+        // load per element, queued behind the previous row's stores.
+        const bool foldRow = prec == CVDV_MIXED && d.lgC == 0 && d.lgHb == 0 && d.lgM >= 13;
+        const FastDiag pre =
+            compile_pre(tile_bits(d), foldRow ? diag_fold_linear(preProg, d.lgInR, d.lgM) : preProg);
         const int usePre = act ? 1 : 0;
         d.tw4 = 0; d.tw4Shift = 0; d.tw4HiBits = 0; d.tw4LogN = 0;
         d.usePre = usePre;
@@ -1445,7 +1450,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     if (total < 4096 || N <= 64) {
         // tiny state: direct DFT, out of place, FP64 accumulation rounded once
         // (per-element table reads: any choice bits)
-        const FastDiag pre = compile_pre(0);
+        const FastDiag pre = compile_pre(0, preProg);
         const int usePre = act ? 1 : 0;
         const double sc = folded ? 1.0 : outScale;
         const double2* twN = (const double2*)full_table(ctx, nb);
@@ -1466,7 +1471,7 @@ static bool fft_run(CVDVContext* ctx, typename CT<R>::type* data, TmpFn tmpFn, u
     if (!four_step_geom(prec, total, sh, nb, f, cfg))
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "register too large for the two-pass FFT"};
     FftPass d1 = f.d1, d2 = f.d2;
-    const FastDiag pre = compile_pre(tile_bits(d1));
+    const FastDiag pre = compile_pre(tile_bits(d1), preProg);
     const int usePre = act ? 1 : 0;
     // inverse: conj(P2(T P1(conj x))) - pass 1 conjugates its input, pass 2 its
     // output; the twiddle T = W_N^{j1 k2} is applied by pass 2 on load, where the
@@ -1529,7 +1534,7 @@ static bool gen_affine_in_rows(const FastDiag& pre, int rowLo, int rowN) {
     for (int i = 0; i < pre.ngen; ++i) {
         const DiagTerm& T = pre.gen[i];
         if (T.ctrl >= rowLo && T.ctrl < rowLo + rowN) return false;
-        if (T.kind == DT_POLY) {
+        if (T.kind == DT_POLY || T.kind == DT_LINF) {
             const int r = rel(T.sh1, T.nb1);
             if (r == 2 || (r == 1 && (T.c2 != 0.0 || T.c3 != 0.0))) return false;
         } else if (T.kind == DT_BILIN) {

@@ -490,11 +490,26 @@ FastDiag compile_diag(CVDVContext* ctx, const DiagProg& p, bool& active, double 
 static uint64_t term_bits(const DiagTerm& t) {
     auto field = [](int sh, int nb) { return (((uint64_t)1 << nb) - 1ull) << sh; };
     uint64_t m;
-    if (t.kind == DT_POLY) m = field(t.sh1, t.nb1);
+    if (t.kind == DT_POLY || t.kind == DT_LINF) m = field(t.sh1, t.nb1);
     else if (t.kind == DT_BILIN) m = field(t.sh1, t.nb1) | field(t.sh2, t.nb2);
     else m = 1ull << t.sh1;
     if (t.ctrl >= 0) m |= 1ull << t.ctrl;
     return m;
+}
+
+DiagProg diag_fold_linear(const DiagProg& p, int sh, int nb) {
+    DiagProg q = p;
+    int ngen = 0;
+    for (int i = 0; i < q.n; ++i) ngen += !term_is_identity(q.t[i]) && !term_tabulated(q.t[i]);
+    for (int i = 0; i < q.n && ngen < CVDV_FAST_GEN; ++i) {
+        DiagTerm& t = q.t[i];
+        if (t.kind == DT_POLY && t.sh1 == sh && t.nb1 == nb && t.c2 == 0.0 && t.c3 == 0.0 && !term_is_identity(t) &&
+            !(t.ctrl >= sh && t.ctrl < sh + nb)) {
+            t.kind = DT_LINF;
+            ++ngen;
+        }
+    }
+    return q;
 }
 
 void fft_settle_for(CVDVContext* ctx, const DiagProg& p, uint64_t except) {

@@ -533,8 +533,9 @@ __device__ __forceinline__ void stage0_load(const C* __restrict__ in, C* x, cons
 //   TM_ANY    tables of several columns stage their product by row in LDS when
 //             every table allows it, else each element takes its flat index;
 //   TM_STAGED every table is read by row (the host checked, tile_tables_by_row())
-//             and is staged by row; generic terms are evaluated one element per
-//             iteration of a rolled loop into the free exchange buffer, which
+//             and is staged by row; generic terms go through the free exchange
+//             buffer (complex64: their row steps per column and row block;
+//             complex128: each element's factor from a rolled loop), which
 //             keeps their FP64 temporaries off the other elements' registers.
 // TM_NONE: the pass applies no phases (only the four-step twiddle, if TW4).
 enum { TM_ROWS = 0, TM_ANY = 1, TM_STAGED = 2, TM_NONE = 3 };
@@ -724,41 +725,58 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         }
         if constexpr (TM == TM_STAGED && LGC > 0) {
             if (pre.ngen > 0) {
-                // One element per iteration: its factor goes to its own slot of the
-                // exchange buffer (element-major, so a wave's writes are
-                // consecutive), then every element takes its product.  complex64:
-                // every generic term is affine in the row index (the host checked,
-                // gen_affine_in_rows()), and a butterfly's rows are q + i M/RP
-                // (column and row block fixed), so its phases are P0 + i D from
-                // two exact FP64 evaluations per butterfly.  complex128: each
-                // element's terms are evaluated exactly.
+                using RR = decltype(C{}.x);
+                if constexpr (sizeof(C) == 8) {
+                    // complex64: every generic term is affine in the row index (the
+                    // host checked, gen_affine_in_rows()).  A butterfly's rows are
+                    // q + i M/RP with its column and row block (position j) fixed,
+                    // so its phases are P(q) + i D_j.  The steps e^{2 pi i i D_j}
+                    // go to the free exchange buffer, RP-1 per position, one
+                    // sin/cos each; a butterfly then takes one FP64 evaluation,
+                    // one sin/cos e0 and RP-1 products e0 * step.
+                    const int nj = (nt * E) >> LGM;
 #pragma unroll 1
-                for (int k = 0; k < NB; ++k) {
-                    if constexpr (sizeof(C) == 8) {
-                        const uint64_t g0 = elem(k, 0), g1 = elem(k, 1);
-                        double p0 = pre.genrev, dl = 0.0;
-                        for (int t = 0; t < pre.ngen; ++t) {
-                            const double v0 = diag_term_rev(P.gen[t], g0);
-                            p0 += cvdv_frac(v0);
-                            dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - v0);
-                        }
-                        dl = cvdv_frac(dl);
+                    for (int s = tt; s < nj * (RP - 1); s += nt) {
+                        const int j = s / (RP - 1), i = s - j * (RP - 1) + 1;
+                        const uint64_t g0 =
+                            inBase + ((uint64_t)(j >> LGC) << d.lgInH2) + (uint64_t)(j & ((1 << LGC) - 1));
+                        const uint64_t g1 = g0 + ((uint64_t)(1 << lgMq) << d.lgInR);
+                        double dl = 0.0;
+                        for (int t = 0; t < pre.ngen; ++t)
+                            dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - diag_term_rev(P.gen[t], g0));
+                        sRow[s] = unit_rev_fast(cvdv_frac((double)i * cvdv_frac(dl)), RR(0));
+                    }
+                    lds_barrier();
+#pragma unroll
+                    for (int k = 0; k < NB; ++k) {
+                        const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+                        const uint64_t g0 = elem(k, 0);
+                        double p0 = pre.genrev;
+                        for (int t = 0; t < pre.ngen; ++t) p0 += cvdv_frac(diag_term_rev(P.gen[t], g0));
+                        const C e0 = unit_rev_fast(cvdv_frac(p0), RR(0));
+                        const C* st = sRow + (p.c | (p.hb << LGC)) * (RP - 1) - 1;
+                        x[k * RP] = cmul(x[k * RP], e0);
+#pragma unroll
+                        for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
+                    }
+                } else {
+                    // complex128: one element per iteration, its terms evaluated
+                    // exactly; its factor goes to its own slot of the exchange
+                    // buffer (element-major, so a wave's writes are consecutive),
+                    // then every element takes its product.
 #pragma unroll 1
-                        for (int i = 0; i < RP; ++i)
-                            sRow[(k * RP + i) * nt + tt] =
-                                unit_rev_fast(cvdv_frac(fma((double)i, dl, p0)), decltype(C{}.x)(0));
-                    } else {
+                    for (int k = 0; k < NB; ++k) {
 #pragma unroll 1
                         for (int i = 0; i < RP; ++i) {
                             const uint64_t g = elem(k, i);
                             double r = pre.genrev;
                             for (int t = 0; t < pre.ngen; ++t) r += cvdv_frac(diag_term_rev(P.gen[t], g));
-                            sRow[(k * RP + i) * nt + tt] = unit_rev_fast(cvdv_frac(r), decltype(C{}.x)(0));
+                            sRow[(k * RP + i) * nt + tt] = unit_rev_fast(cvdv_frac(r), RR(0));
                         }
                     }
-                }
 #pragma unroll
-                for (int j = 0; j < E; ++j) x[j] = cmul(x[j], sRow[j * nt + tt]);
+                    for (int j = 0; j < E; ++j) x[j] = cmul(x[j], sRow[j * nt + tt]);
+                }
                 lds_barrier();  // the stages that follow write other threads' slots
             }
         } else if (pre.ngen > 0) {

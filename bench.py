@@ -236,17 +236,44 @@ def run_c4(prec, circuits, n_seq, steps):
             "probs0": probs, "norms": norms}
 
 
+# Sources each counter summary's kernels are compiled from (the phase programs
+# they apply come from gates.hip).  A summary records their fingerprint; a figure
+# is read from it only while the sources still match.
+FFT_SOURCES = ["fft.hip", "fft_bfly.h", "diag_fast.h", "cvdv_internal.h", "gates.hip"]
+BATCH_SOURCES = ["batch.hip", "fft_bfly.h", "diag_fast.h", "cvdv_internal.h", "gates.hip"]
+
+
+def source_fingerprint(names):
+    """sha256 (first 16 hex digits) of the named files of cuda-cvdv_amd/csrc, in order."""
+    import hashlib
+    h = hashlib.sha256()
+    for n in names:
+        with open(os.path.join(ROOT, "cuda-cvdv_amd", "csrc", n), "rb") as f:
+            h.update(n.encode() + b"\0" + f.read())
+    return h.hexdigest()[:16]
+
+
+def profile_figure(fname, sources, key):
+    """(value, note): the figure `key` of the committed counter summary
+    profiles/<fname>, or None with the reason (missing, or taken on other sources)."""
+    try:
+        with open(os.path.join(ROOT, "profiles", fname)) as f:
+            lines = f.read().splitlines()
+    except OSError:
+        return None, f"profiles/{fname} missing"
+    fp = next((l.split("=")[1].split()[0] for l in lines if l.startswith("sources_sha256 =")), None)
+    if fp != source_fingerprint(sources):
+        return None, f"profiles/{fname} was taken on other kernel sources (fingerprint {fp})"
+    for l in lines:
+        if l.startswith(key + " ="):
+            return float(l.split("=")[1].split()[0]), None
+    return None, f"{key} not in profiles/{fname}"
+
+
 def c4_pmc(key):
     """A figure of the batch kernel from the committed counter summary of the C4
     workload (profiles/k_batch_c4_pmc.txt, separate rocprofv3 runs), or None."""
-    try:
-        with open(os.path.join(ROOT, "profiles", "k_batch_c4_pmc.txt")) as f:
-            for line in f:
-                if line.startswith(key + " ="):
-                    return float(line.split("=")[1].split()[0])
-    except OSError:
-        pass
-    return None
+    return profile_figure("k_batch_c4_pmc.txt", BATCH_SOURCES, key)[0]
 
 
 def c4_roofline(prec, r, circuits):
@@ -263,9 +290,9 @@ def c4_roofline(prec, r, circuits):
     alg = out_b + in_b
     ach = alg / (r["kernel_ms"] * 1e-3) / 1e9
     tag = "c64" if prec == "mixed" else "c128"
-    traffic = c4_pmc(f"traffic_bytes_per_launch[{tag}]")
+    traffic, note = profile_figure("k_batch_c4_pmc.txt", BATCH_SOURCES, f"traffic_bytes_per_launch[{tag}]")
     return {"bound": "latency", "algorithmic_hbm_bytes": alg, "achieved_hbm_gbps": ach, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+            "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": note,
             "traffic_gbps": traffic / (r["kernel_ms"] * 1e-3) / 1e9 if traffic else None,
             "valu_issue": c4_pmc(f"valu_issue_frac[{tag}]"), "kernel_ms": r["kernel_ms"]}
 
@@ -856,21 +883,13 @@ def qft_error_envelope(qubits=range(5, 13)):
 
 
 def pmc_traffic(kind):
-    """HBM bytes per launch of the FFT pass kind `kind` (fft_row, fft_half,
+    """(HBM bytes per launch, note) of the FFT pass kind `kind` (fft_row, fft_half,
     fft_fused) from the committed counter summary of the [14,14] complex64
     Gaussian circuit (profiles/fft_c2_mixed_pmc.txt: FETCH_SIZE and WRITE_SIZE
-    per kernel, each scaled by the same counter on a plain copy of known size),
-    or None.  The counters come from separate rocprofv3 runs, not from this one."""
-    path = os.path.join(ROOT, "profiles", "fft_c2_mixed_pmc.txt")
-    key = f"traffic_bytes_per_launch[{kind}]"
-    try:
-        with open(path) as f:
-            for line in f:
-                if line.startswith(key):
-                    return float(line.split("=")[1].split()[0])
-    except OSError:
-        pass
-    return None
+    per kernel, each scaled by the same counter on a plain copy of known size);
+    None and the reason when the summary is missing or was taken on other kernel
+    sources.  The counters come from separate rocprofv3 runs, not from this one."""
+    return profile_figure("fft_c2_mixed_pmc.txt", FFT_SOURCES, f"traffic_bytes_per_launch[{kind}]")
 
 
 # FFT pass kinds of the profile (cvdvProfileRead) -> kernel name and the register
@@ -893,8 +912,9 @@ def roofline(prof, pass_bytes, copy):
     avg = ms / n
     alg = per * pass_bytes
     achieved = alg / (avg * 1e-3) / 1e9
+    traffic, traffic_note = pmc_traffic(dom)
     out = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-           "frac": achieved / HBM_PEAK_GBS, "traffic": pmc_traffic(dom),
+           "frac": achieved / HBM_PEAK_GBS, "traffic": traffic, "traffic_note": traffic_note,
            "kernel": name, "launches": n, "avg_launch_ms": avg, "algorithmic_bytes_per_launch": alg,
            "copy_gbps": copy["gbps"], "frac_of_copy": achieved / copy["gbps"], "share_of_fft_ms": ms / prof["fft_ms"],
            "fft_kinds": {k: {"kernel": FFT_KINDS[k][0], "launches": prof[k + "_launches"], "ms": prof[k + "_ms"],

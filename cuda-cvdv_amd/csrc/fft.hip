@@ -297,6 +297,22 @@ struct FftPass {
     double outScale;  // applied as exact v*hi + rounded v*lo (one FMA): an FP32 store stays unbiased
 };
 
+// The value lane `lane` of the wave holds (`lane` the same on every lane, and
+// that lane active).
+__device__ __forceinline__ float2 lane_value(float2 v, int lane) {
+    return make_float2(__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), lane)),
+                       __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), lane)));
+}
+__device__ __forceinline__ double2 lane_value(double2 v, int lane) {
+    auto rd = [lane](double d) {
+        const long long b = __double_as_longlong(d);
+        const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, lane);
+        const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), lane);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    };
+    return make_double2(rd(v.x), rd(v.y));
+}
+
 // LDS index with one pad slot per 32 entries (breaks power-of-two strides)
 __device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
 
@@ -790,8 +806,17 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 // times the tile-wide steps e^{2 pi i a 2 i M/RP}: one sin/cos per
                 // butterfly, and RP-1 per tile (uniform across the wave).
                 C st[RP];
+                if (nt >= 64) {
+                    // full waves: lane i evaluates step i, the others read it
+                    const int li = (int)__builtin_amdgcn_mbcnt_lo(~0u, 0u) & (RP - 1);
+                    const C mine = unit_rev_fast(cvdv_frac(gf.a * (double)(2 * (li << lgMq))), R(0));
 #pragma unroll
-                for (int i = 1; i < RP; ++i) st[i] = unit_rev_fast(cvdv_frac(gf.a * (double)(2 * (i << lgMq))), R(0));
+                    for (int i = 1; i < RP; ++i) st[i] = lane_value(mine, i);
+                } else {
+#pragma unroll
+                    for (int i = 1; i < RP; ++i)
+                        st[i] = unit_rev_fast(cvdv_frac(gf.a * (double)(2 * (i << lgMq))), R(0));
+                }
 #pragma unroll
                 for (int k = 0; k < NB; ++k) {
                     const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
@@ -892,7 +917,10 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ PreLds P;
     C* lds = reinterpret_cast<C*>(smem_raw);
-    const int tid = threadIdx.x, nt = blockDim.x;
+    // persistent one-row tiles are launched with 2^LGM / E threads (the host
+    // checks): a compile-time count lets the butterflies of one thread, which sit
+    // nt apart, share the twiddles their rows have in common
+    const int tid = threadIdx.x, nt = PF ? (1 << LGM) / E : (int)blockDim.x;
     constexpr int lgC = LGC;  // the tile's column count is part of the instance
     const int lgHb = d.lgHb;
     const int tile = 1 << (LGM + lgC + lgHb);
@@ -1296,6 +1324,8 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         auto go = [&](auto kfn, int E, bool persistent) {
             if (tile / E > tile_threads_max<R, LGM, LGC>())
                 throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile larger than its kernel's workgroup bound"};
+            if (persistent && tile != (1 << LGM))
+                throw CvdvError{CVDV_ERR_UNSUPPORTED, "persistent FFT tile of more than one row"};
             if (lds > 64 * 1024)
                 HIPCHK(hipFuncSetAttribute((const void*)kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
             // persistent instances: as many workgroups as the device holds at once

@@ -313,16 +313,33 @@ __device__ __forceinline__ double2 lane_value(double2 v, int lane) {
     return make_double2(rd(v.x), rd(v.y));
 }
 
-// LDS index with one pad slot per 32 entries (breaks power-of-two strides)
-__device__ __forceinline__ int lds_ix(int i) { return i + (i >> 5); }
+// LDS layouts of the Stockham exchanges (padding breaks power-of-two strides).
+// LAY 0, every exchange of every tile unless below: one pad entry per 32.
+// One-row complex64 tiles of 2^13-2^14 points write exchange 0 at a stride of 8
+// entries and exchange 1 in runs of 8 lanes 64 entries apart, which puts two
+// lanes of every 16 on one LDS bank under LAY 0: there exchange 0 takes one pad
+// entry per 16 (LAY 1) and exchange 1 eight per 64 (LAY 2).
+template <int LAY>
+__device__ __forceinline__ int lay_ix(int i) {
+    return LAY == 0 ? i + (i >> 5) : (LAY == 1 ? i + (i >> 4) : i + ((i >> 6) << 3));
+}
+template <int LAY> constexpr int lay_period() { return LAY == 0 ? 32 : (LAY == 1 ? 16 : 64); }
+template <int LAY> constexpr int lay_padded(int stride) {
+    return stride + (LAY == 0 ? stride / 32 : (LAY == 1 ? stride / 16 : stride / 8));
+}
+// The layout of exchange XS (stage XS's outputs, read by stage XS + 1).
+template <bool ONE, int XS> constexpr int ex_lay() { return !ONE ? 0 : (XS == 0 ? 1 : (XS == 1 ? 2 : 0)); }
 
-// Padded index of i0 + m * STRIDE.  When STRIDE is a multiple of 32 the padding
-// is linear (adding it leaves the low five bits of i0 alone), so every element
-// is the padded base plus a compile-time offset (an immediate in the LDS access).
-template <int STRIDE>
+__device__ __forceinline__ int lds_ix(int i) { return lay_ix<0>(i); }
+
+// Padded index of i0 + m * STRIDE.  When STRIDE is a multiple of the layout's
+// period the padding is linear (adding it leaves the bits below the period
+// alone), so every element is the padded base plus a compile-time offset (an
+// immediate in the LDS access).
+template <int STRIDE, int LAY = 0>
 __device__ __forceinline__ int lds_at(int i0, int pbase, int m) {
-    if constexpr (STRIDE % 32 == 0) return pbase + m * (STRIDE + STRIDE / 32);
-    else return lds_ix(i0 + m * STRIDE);
+    if constexpr (STRIDE % lay_period<LAY>() == 0) return pbase + m * lay_padded<LAY>(STRIDE);
+    else return lay_ix<LAY>(i0 + m * STRIDE);
 }
 
 // Stage plan of a length-2^LGM transform: radix-8 stages, then one radix-4 or
@@ -426,9 +443,11 @@ __device__ __forceinline__ void stage_compute(C* x, const C* __restrict__ tw, in
     }
 }
 
-// Outputs of stage S (radix RP) -> LDS in Stockham order.
-template <int RP, int E, int LGM, int LGC, int S, typename C>
+// Outputs of stage S (radix RP) -> LDS in Stockham order (exchange S; ONE: the
+// one-row layouts).
+template <int RP, int E, int LGM, int LGC, int S, bool ONE = false, typename C>
 __device__ __forceinline__ void stage_write(C* lds, const C* x, int tid, int nt) {
+    constexpr int LAY = ex_lay<ONE, S>();
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgL = 3 * S;
@@ -437,15 +456,23 @@ __device__ __forceinline__ void stage_write(C* lds, const C* x, int tid, int nt)
     for (int k = 0; k < NB; ++k) {
         const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
         const int ob = (p.hb << LGM) + ((p.q >> lgL) << (LGR + lgL)) + (p.q & ((1 << lgL) - 1));
-        const int i0 = (ob << LGC) + p.c, pb = lds_ix(i0);
+        const int i0 = (ob << LGC) + p.c, pb = lay_ix<LAY>(i0);
+        if constexpr (LAY != 0) {
+            // one-row exchanges 0 and 1: i0 = 8q with entries i0 + m (period 16),
+            // or i0 mod 64 < 8 with entries i0 + 8m (period 64): all within i0's period
 #pragma unroll
-        for (int m = 0; m < RP; ++m) lds[lds_at<STRIDE>(i0, pb, m)] = x[k * RP + m];
+            for (int m = 0; m < RP; ++m) lds[pb + m * STRIDE] = x[k * RP + m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < RP; ++m) lds[lds_at<STRIDE, LAY>(i0, pb, m)] = x[k * RP + m];
+        }
     }
 }
 
-// Inputs of a stage of radix RP <- LDS.
-template <int RP, int E, int LGM, int LGC, typename C>
+// Inputs of a stage of radix RP <- LDS (exchange XS).
+template <int RP, int E, int LGM, int LGC, int XS = 0, bool ONE = false, typename C>
 __device__ __forceinline__ void stage_read(const C* lds, C* x, int tid, int nt) {
+    constexpr int LAY = ex_lay<ONE, XS>();
     constexpr int NB = E / RP;
     constexpr int LGR = RP == 8 ? 3 : (RP == 4 ? 2 : 1);
     constexpr int lgMq = LGM - LGR;
@@ -453,9 +480,9 @@ __device__ __forceinline__ void stage_read(const C* lds, C* x, int tid, int nt) 
 #pragma unroll
     for (int k = 0; k < NB; ++k) {
         const BflyPos p = bfly_pos<RP, LGM>(tid + k * nt, LGC);
-        const int i0 = (((p.hb << LGM) + p.q) << LGC) + p.c, pb = lds_ix(i0);
+        const int i0 = (((p.hb << LGM) + p.q) << LGC) + p.c, pb = lay_ix<LAY>(i0);
 #pragma unroll
-        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_at<STRIDE>(i0, pb, i)];
+        for (int i = 0; i < RP; ++i) x[k * RP + i] = lds[lds_at<STRIDE, LAY>(i0, pb, i)];
     }
 }
 
@@ -883,18 +910,25 @@ __device__ __forceinline__ void last_store(C* __restrict__ out, const C* v, cons
 
 // Stages 1 .. NST-1: exchange the previous stage's outputs through LDS, then
 // twiddle and transform.
-template <int LGM, int LGC, int S, int E, bool TWL, typename C>
+template <int LGM, int LGC, int S, int E, bool TWL, bool ONE = false, typename C>
 __device__ __forceinline__ void run_stages(C* x, C* lds, const C* __restrict__ tw, int tid, int nt) {
     if constexpr (S < Plan<LGM>::NST) {
         constexpr int RPprev = 1 << Plan<LGM>::lgr(S - 1);
         constexpr int RP = 1 << Plan<LGM>::lgr(S);
-        stage_write<RPprev, E, LGM, LGC, S - 1>(lds, x, tid, nt);
+        stage_write<RPprev, E, LGM, LGC, S - 1, ONE>(lds, x, tid, nt);
         lds_barrier();
-        stage_read<RP, E, LGM, LGC>(lds, x, tid, nt);
+        stage_read<RP, E, LGM, LGC, S - 1, ONE>(lds, x, tid, nt);
         lds_barrier();
         stage_compute<RP, E, LGM, LGC, S, TWL>(x, tw, tid, nt);
-        run_stages<LGM, LGC, S + 1, E, TWL>(x, lds, tw, tid, nt);
+        run_stages<LGM, LGC, S + 1, E, TWL, ONE>(x, lds, tw, tid, nt);
     }
+}
+
+// One-row complex64 tiles of 2^13-2^14 points lay out their first two exchanges
+// apart from the others (ex_lay); their LDS ends at tile + tile/8.
+template <typename R, int LGM, int LGC>
+constexpr bool one_row_layouts() {
+    return sizeof(R) == 4 && LGC == 0 && LGM >= 13 && Plan<LGM>::lgr(0) == 3 && Plan<LGM>::lgr(1) == 3;
 }
 
 // PF: persistent workgroups that prefetch the next tile (one-row tiles).
@@ -924,6 +958,10 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     constexpr int lgC = LGC;  // the tile's column count is part of the instance
     const int lgHb = d.lgHb;
     const int tile = 1 << (LGM + lgC + lgHb);
+    // one-row complex64 tiles of 2^13-2^14 points: exchanges 0 and 1 in their own
+    // layouts, the largest of which ends at tile + tile/8 (the host sizes LDS so)
+    constexpr bool ONE = one_row_layouts<R, LGM, LGC>();
+    const int ldsEnd = ONE ? tile + (tile >> 3) : lds_ix(tile);
 
     if (d.usePre) {
         // constant indices: the argument is read in place (one entry per thread,
@@ -952,7 +990,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     // no global loads remain between a prefetch and its use.
     const C* twl = tw;
     if constexpr (PF) {
-        C* sTw = lds + lds_ix(tile);
+        C* sTw = lds + ldsEnd;
         for (int e = tid; e < SplitPlan<LGM>::total(); e += nt) sTw[e] = tw[e];
         __syncthreads();
         twl = sTw;
@@ -976,7 +1014,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
     // Transform the phased tile in registers (the last stage's outputs stay in x).
     auto stages = [&](C* x, int ti, const C* twt) {
         stage_compute<R0, E, LGM, LGC, 0, PF>(x, twt, ti, nt);
-        run_stages<LGM, LGC, 1, E, PF>(x, lds, twt, ti, nt);
+        run_stages<LGM, LGC, 1, E, PF, ONE>(x, lds, twt, ti, nt);
     };
     // Store the transformed tile t.
     // Staged store, first half: the last stage's outputs into LDS in natural order.
@@ -1064,7 +1102,7 @@ k_fft_tile(const typename CT<R>::type* in, typename CT<R>::type* out, FftPass d,
             // addressed from the shared array so that their loads stay LDS loads
             // (a generic pointer would also count on the memory counter).
             int ti = tid;
-            int twOff = lds_ix(tile);
+            int twOff = ldsEnd;
             asm volatile("" : "+v"(ti), "+v"(twOff));
             const C* twt = lds + twOff;
             C x[E];
@@ -1341,10 +1379,17 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
         // read by row run one tile per workgroup)
         constexpr bool PF = LGC == 0 && LGM >= 13 && sizeof(R) == 4;
         const bool pf = PF && !d.tw4 && !d.conjLoad && (!d.usePre || table_by_row(pre, d.lgInR, LGM));
+        if constexpr (one_row_layouts<R, LGM, LGC>()) {
+            if (tile != (1 << LGM))
+                throw CvdvError{CVDV_ERR_UNSUPPORTED, "one-row FFT tile of more than one row"};
+            lds = (size_t)(tile + tile / 8) * sizeof(typename CT<R>::type);
+        }
         if (pf) {
             tw = (const typename CT<R>::type*)split_table(ctx, LGM, sizeof(R) == 4 ? CVDV_MIXED : CVDV_C128);
             lds += (size_t)SplitPlan<LGM>::total() * sizeof(typename CT<R>::type);
         }
+        if (lds > device_max_lds())
+            throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile needs more LDS than the device has"};
         if constexpr (LGM == 14 && LGC == 0 && sizeof(R) == 4) {
             if (tile / 8 > 1024) {
                 if (pf) go(k_fft_tile<R, LGM, LGC, 16, PF>, 16, PF);

@@ -295,6 +295,7 @@ struct FftPass {
     int lgInH1, lgInH2, lgInR;     // every stride is a power of two
     int lgOutH1, lgOutH2, lgOutR;
     double outScale;  // applied as exact v*hi + rounded v*lo (one FMA): an FP32 store stays unbiased
+    int genSteps;     // 1: every generic term is affine in the tile's row index (the host checked)
 };
 
 // The value lane `lane` of the wave holds (`lane` the same on every lane, and
@@ -726,6 +727,43 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
                 }
             }
         };
+        // Generic terms affine in the row index (complex64 staged instances always;
+        // otherwise when d.genSteps): a butterfly's rows are q + i M/RP with its
+        // column and row block (position j) fixed, so its phases are P(q) + i D_j.
+        // The steps e^{2 pi i i D_j} go to the free exchange buffer, RP-1 per
+        // position, one sin/cos each; a butterfly then takes one FP64 evaluation,
+        // one sin/cos e0 and RP-1 products e0 * step.  The caller puts a barrier
+        // before the buffer is written again.
+        auto row_steps = [&]() {
+            if constexpr (LGC > 0) {
+                using RR = decltype(C{}.x);
+                const int nj = (nt * E) >> LGM;
+#pragma unroll 1
+                for (int s = tt; s < nj * (RP - 1); s += nt) {
+                    const int j = s / (RP - 1), i = s - j * (RP - 1) + 1;
+                    const uint64_t g0 =
+                        inBase + ((uint64_t)(j >> LGC) << d.lgInH2) + (uint64_t)(j & ((1 << LGC) - 1));
+                    const uint64_t g1 = g0 + ((uint64_t)(1 << lgMq) << d.lgInR);
+                    double dl = 0.0;
+                    for (int t = 0; t < pre.ngen; ++t)
+                        dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - diag_term_rev(P.gen[t], g0));
+                    sRow[s] = unit_rev_fast(cvdv_frac((double)i * cvdv_frac(dl)), RR(0));
+                }
+                lds_barrier();
+#pragma unroll
+                for (int k = 0; k < NB; ++k) {
+                    const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+                    const uint64_t g0 = elem(k, 0);
+                    double p0 = pre.genrev;
+                    for (int t = 0; t < pre.ngen; ++t) p0 += cvdv_frac(diag_term_rev(P.gen[t], g0));
+                    const C e0 = unit_rev_fast(cvdv_frac(p0), RR(0));
+                    const C* st = sRow + (p.c | (p.hb << LGC)) * (RP - 1) - 1;
+                    x[k * RP] = cmul(x[k * RP], e0);
+#pragma unroll
+                    for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
+                }
+            }
+        };
         bool staged = false;
         if constexpr (LGC > 0 && TM != TM_ROWS) {
             // Tiles of several columns: when every table is read by row, their
@@ -769,44 +807,14 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         if constexpr (TM == TM_STAGED && LGC > 0) {
             if (pre.ngen > 0) {
                 using RR = decltype(C{}.x);
-                if constexpr (sizeof(C) == 8) {
-                    // complex64: every generic term is affine in the row index (the
-                    // host checked, gen_affine_in_rows()).  A butterfly's rows are
-                    // q + i M/RP with its column and row block (position j) fixed,
-                    // so its phases are P(q) + i D_j.  The steps e^{2 pi i i D_j}
-                    // go to the free exchange buffer, RP-1 per position, one
-                    // sin/cos each; a butterfly then takes one FP64 evaluation,
-                    // one sin/cos e0 and RP-1 products e0 * step.
-                    const int nj = (nt * E) >> LGM;
-#pragma unroll 1
-                    for (int s = tt; s < nj * (RP - 1); s += nt) {
-                        const int j = s / (RP - 1), i = s - j * (RP - 1) + 1;
-                        const uint64_t g0 =
-                            inBase + ((uint64_t)(j >> LGC) << d.lgInH2) + (uint64_t)(j & ((1 << LGC) - 1));
-                        const uint64_t g1 = g0 + ((uint64_t)(1 << lgMq) << d.lgInR);
-                        double dl = 0.0;
-                        for (int t = 0; t < pre.ngen; ++t)
-                            dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - diag_term_rev(P.gen[t], g0));
-                        sRow[s] = unit_rev_fast(cvdv_frac((double)i * cvdv_frac(dl)), RR(0));
-                    }
-                    lds_barrier();
-#pragma unroll
-                    for (int k = 0; k < NB; ++k) {
-                        const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
-                        const uint64_t g0 = elem(k, 0);
-                        double p0 = pre.genrev;
-                        for (int t = 0; t < pre.ngen; ++t) p0 += cvdv_frac(diag_term_rev(P.gen[t], g0));
-                        const C e0 = unit_rev_fast(cvdv_frac(p0), RR(0));
-                        const C* st = sRow + (p.c | (p.hb << LGC)) * (RP - 1) - 1;
-                        x[k * RP] = cmul(x[k * RP], e0);
-#pragma unroll
-                        for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
-                    }
+                if (sizeof(C) == 8 || d.genSteps) {
+                    row_steps();
                 } else {
-                    // complex128: one element per iteration, its terms evaluated
-                    // exactly; its factor goes to its own slot of the exchange
-                    // buffer (element-major, so a wave's writes are consecutive),
-                    // then every element takes its product.
+                    // complex128 with terms that are not affine in the rows: one
+                    // element per iteration, its terms evaluated exactly; its factor
+                    // goes to its own slot of the exchange buffer (element-major, so
+                    // a wave's writes are consecutive), then every element takes its
+                    // product.
 #pragma unroll 1
                     for (int k = 0; k < NB; ++k) {
 #pragma unroll 1
@@ -853,6 +861,9 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
 #pragma unroll
                     for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
                 }
+            } else if (LGC > 0 && d.genSteps) {
+                row_steps();
+                lds_barrier();  // the stages that follow write the exchange buffer
             } else {
 #pragma unroll
                 for (int k = 0; k < NB; ++k)
@@ -1354,11 +1365,15 @@ static bool table_by_row(const FastDiag& pre, int fs, int fn) {
     return true;
 }
 
+static bool gen_affine_in_rows(const FastDiag& pre, int rowLo, int rowN);
+
 template <typename R, int LGM, int LGC>
 static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typename CT<R>::type* out,
                         const FftPass& d, const FastDiag& pre, const typename CT<R>::type* tw,
                         const typename CT<R>::type* tw4, uint64_t blocks, int tile, size_t lds) {
     if constexpr (tile_instance<R, LGM, LGC>()) {
+        FftPass dg = d;
+        dg.genSteps = d.usePre && pre.ngen > 0 && gen_affine_in_rows(pre, d.lgInR, LGM) ? 1 : 0;
         auto go = [&](auto kfn, int E, bool persistent) {
             if (tile / E > tile_threads_max<R, LGM, LGC>())
                 throw CvdvError{CVDV_ERR_UNSUPPORTED, "FFT tile larger than its kernel's workgroup bound"};
@@ -1369,7 +1384,7 @@ static void launch_geom(CVDVContext* ctx, const typename CT<R>::type* in, typena
             // persistent instances: as many workgroups as the device holds at once
             const uint64_t grid =
                 persistent ? std::min<uint64_t>(blocks, resident_blocks((const void*)kfn, tile / E, lds)) : blocks;
-            hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(tile / E), lds, ctx->stream, in, out, d, pre, tw,
+            hipLaunchKernelGGL(kfn, dim3((unsigned)grid), dim3(tile / E), lds, ctx->stream, in, out, dg, pre, tw,
                                tw4);
         };
         // one-row complex64 tiles that fill a CU's LDS alone prefetch the next row
@@ -1652,16 +1667,18 @@ static void launch_fused_geom(CVDVContext* ctx, typename CT<R>::type* data, cons
         // complex64, whose generic terms are affine in the row index) take the
         // instance that stages them through LDS: fewer registers, more
         // workgroups per CU
+        FftPass dDg = dD;
+        dDg.genSteps = dD.usePre && pre.ngen > 0 && gen_affine_in_rows(pre, dD.lgInR, LGM) ? 1 : 0;
         if constexpr (LGC > 0) {
-            const bool gen = sizeof(R) == 8 || pre.ngen == 0 || gen_affine_in_rows(pre, dD.lgInR, LGM);
+            const bool gen = sizeof(R) == 8 || pre.ngen == 0 || dDg.genSteps;
             if (!dD.usePre || (tile_tables_by_row(pre, dD) && gen)) {
                 hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8, true>), dim3((unsigned)blocks), dim3(tile / 8), lds,
-                                   ctx->stream, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
+                                   ctx->stream, data, d, dDg, pre, conjMid, swapSh, swapN, tw, tw4);
                 return;
             }
         }
         hipLaunchKernelGGL((k_fft_fused<R, LGM, LGC, 8, false>), dim3((unsigned)blocks), dim3(tile / 8), lds,
-                           ctx->stream, data, d, dD, pre, conjMid, swapSh, swapN, tw, tw4);
+                           ctx->stream, data, d, dDg, pre, conjMid, swapSh, swapN, tw, tw4);
     } else {
         throw CvdvError{CVDV_ERR_UNSUPPORTED, "fused FFT tile shape without a kernel instance"};
     }

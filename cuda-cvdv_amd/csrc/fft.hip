@@ -735,33 +735,31 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
         // one sin/cos e0 and RP-1 products e0 * step.  The caller puts a barrier
         // before the buffer is written again.
         auto row_steps = [&]() {
-            if constexpr (LGC > 0) {
-                using RR = decltype(C{}.x);
-                const int nj = (nt * E) >> LGM;
+            using RR = decltype(C{}.x);
+            const int nj = (nt * E) >> LGM;
 #pragma unroll 1
-                for (int s = tt; s < nj * (RP - 1); s += nt) {
-                    const int j = s / (RP - 1), i = s - j * (RP - 1) + 1;
-                    const uint64_t g0 =
-                        inBase + ((uint64_t)(j >> LGC) << d.lgInH2) + (uint64_t)(j & ((1 << LGC) - 1));
-                    const uint64_t g1 = g0 + ((uint64_t)(1 << lgMq) << d.lgInR);
-                    double dl = 0.0;
-                    for (int t = 0; t < pre.ngen; ++t)
-                        dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - diag_term_rev(P.gen[t], g0));
-                    sRow[s] = unit_rev_fast(cvdv_frac((double)i * cvdv_frac(dl)), RR(0));
-                }
-                lds_barrier();
+            for (int s = tt; s < nj * (RP - 1); s += nt) {
+                const int j = s / (RP - 1), i = s - j * (RP - 1) + 1;
+                const uint64_t g0 =
+                    inBase + ((uint64_t)(j >> LGC) << d.lgInH2) + (uint64_t)(j & ((1 << LGC) - 1));
+                const uint64_t g1 = g0 + ((uint64_t)(1 << lgMq) << d.lgInR);
+                double dl = 0.0;
+                for (int t = 0; t < pre.ngen; ++t)
+                    dl += cvdv_frac(diag_term_rev(P.gen[t], g1) - diag_term_rev(P.gen[t], g0));
+                sRow[s] = unit_rev_fast(cvdv_frac((double)i * cvdv_frac(dl)), RR(0));
+            }
+            lds_barrier();
 #pragma unroll
-                for (int k = 0; k < NB; ++k) {
-                    const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
-                    const uint64_t g0 = elem(k, 0);
-                    double p0 = pre.genrev;
-                    for (int t = 0; t < pre.ngen; ++t) p0 += cvdv_frac(diag_term_rev(P.gen[t], g0));
-                    const C e0 = unit_rev_fast(cvdv_frac(p0), RR(0));
-                    const C* st = sRow + (p.c | (p.hb << LGC)) * (RP - 1) - 1;
-                    x[k * RP] = cmul(x[k * RP], e0);
+            for (int k = 0; k < NB; ++k) {
+                const BflyPos p = bfly_pos<RP, LGM>(tt + k * nt, LGC);
+                const uint64_t g0 = elem(k, 0);
+                double p0 = pre.genrev;
+                for (int t = 0; t < pre.ngen; ++t) p0 += cvdv_frac(diag_term_rev(P.gen[t], g0));
+                const C e0 = unit_rev_fast(cvdv_frac(p0), RR(0));
+                const C* st = sRow + (p.c | (p.hb << LGC)) * (RP - 1) - 1;
+                x[k * RP] = cmul(x[k * RP], e0);
 #pragma unroll
-                    for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
-                }
+                for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
             }
         };
         bool staged = false;
@@ -861,7 +859,8 @@ __device__ __forceinline__ void stage0_pre(C* x, const FftPass& d, uint64_t inBa
 #pragma unroll
                     for (int i = 1; i < RP; ++i) x[k * RP + i] = cmul(x[k * RP + i], cmul(e0, st[i]));
                 }
-            } else if (LGC > 0 && d.genSteps) {
+            } else if (TM != TM_ROWS && d.genSteps) {
+                // tiles of several columns or row blocks
                 row_steps();
                 lds_barrier();  // the stages that follow write the exchange buffer
             } else {

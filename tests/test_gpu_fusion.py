@@ -76,8 +76,10 @@ def _run(regs, init, ops, prec, fuse=True, reads=()):
 # strided registers served by the fused halves: [1,10,4] (2^10 at stride 16: row
 # blocks, per-butterfly twiddles), [2,12,6] (2^12 at stride 64: one row block),
 # [1,10,10] (the headline's shape at 2^10), [2,14] complex128 (contiguous 2^14 in
-# two passes: twiddles per column)
-SHAPES = [([1, 10, 4], 31), ([2, 12, 6], 32), ([1, 10, 10], 33), ([2, 14], 34)]
+# two passes: twiddles per column), [1,4,14] (complex128: the contiguous 2^14
+# register's one-column tiles over several row blocks, with beam-splitter and
+# q1q2 terms from the small oscillator)
+SHAPES = [([1, 10, 4], 31), ([2, 12, 6], 32), ([1, 10, 10], 33), ([2, 14], 34), ([1, 4, 14], 35)]
 
 
 @pytest.mark.parametrize("prec", PRECS)

@@ -781,18 +781,19 @@ def dft_gemm_probe(reps=5):
     return out
 
 
-def _fock_table(N):
+def _fock_table(N, shift=0.0, scale=1.0, pref=1.0):
     """Grid-sampled Hermite functions |n>, n < N (separable.py:62-74's recurrence
-    for every n at once), FP64.  The recurrence runs on mantissas m_n(x) with a
+    for every n at once), FP64, at the coordinates x*scale - shift, times pref (the
+    analytic displaced and squeezed Fock states of the gate studies).  The recurrence runs on mantissas m_n(x) with a
     per-point log scale l(x) (h_n = m_n e^l, l starting at -x^2/2), renormalised
     by 2^-500 when a mantissa passes 2^500, and each row is formed as
     sign(m) e^(log|m| + l): e^(-x^2/2) alone underflows for x^2/2 > 745, which
     would cut the tails of every |n> above n ~ 745 (grids of 2^11 points and up)."""
     dx = math.sqrt(2 * math.pi / N)
-    x = (np.arange(N) - (N - 1) / 2) * dx
+    x = (np.arange(N) - (N - 1) / 2) * dx * scale - shift
     F = np.empty((N, N))
     lscale = -0.5 * x * x
-    cur = np.full(N, math.pi ** -0.25 * math.sqrt(dx))
+    cur = np.full(N, pref * math.pi ** -0.25 * math.sqrt(dx))
     prev = np.zeros(N)
 
     def row(m):
@@ -809,6 +810,21 @@ def _fock_table(N):
             lscale = lscale + np.where(big, 500 * math.log(2.0), 0.0)
         prev, cur = cur, nxt
         F[k] = row(cur)
+    return F
+
+
+def _fock_table_plain(N):
+    """The same Hermite functions by the plain recurrence from e^(-x^2/2), as the
+    reference's analysis/_common.py:37-62 computes them: at N = 1024 the outermost
+    points underflow to zero for every n."""
+    dx = math.sqrt(2 * math.pi / N)
+    x = (np.arange(N) - (N - 1) / 2) * dx
+    F = np.empty((N, N))
+    prev, cur = np.zeros(N), np.exp(-0.5 * x * x) * math.pi ** -0.25 * math.sqrt(dx)
+    F[0] = cur
+    for k in range(1, N):
+        prev, cur = cur, math.sqrt(2.0 / k) * x * cur - math.sqrt((k - 1.0) / k) * prev
+        F[k] = cur
     return F
 
 
@@ -883,6 +899,130 @@ def qft_error_envelope(qubits=range(5, 13)):
     return out
 
 
+# The reference's other discretisation-error studies (ERROR-ANALYSIS.md:19-71,
+# analysis/{comm,disp,rot,squeeze,beam_splitter}_err.py): grid sizes, gate
+# parameters, the 0.1 stop and the 1e-9 fit cut-off as there.
+GATE_STUDIES = {
+    "commutator": {"N": [32, 64, 128, 256, 512, 1024]},
+    # the same with the reference's underflowing Fock table (differs at N = 1024)
+    "commutator_reference_table": {"N": [32, 64, 128, 256, 512, 1024], "plain": True},
+    "displacement": {"N": [64, 128, 256, 512], "alpha": 4.0},
+    "rotation": {"N": [64, 128, 256, 512], "theta": math.pi / 4},
+    "squeeze": {"N": [64, 128, 256, 512], "r": 1.0, "min_n": 4},
+    "beam_splitter": {"N": [32, 64, 128, 256], "theta": math.pi / 2, "min_n": 4},
+}
+
+
+def _gate_study_eps(study, N, prec):
+    """Per-Fock errors eps_n of one study at grid size N: every Fock state at once
+    as the slices of a state whose first register indexes n."""
+    from hipcvdv import HipCvdv
+    import torch
+
+    q = N.bit_length() - 1
+    par = GATE_STUDIES[study]
+    F = _fock_table_plain(N) if par.get("plain") else _fock_table(N)
+
+    def run(regs, arr, gate):
+        h = HipCvdv(regs, precision=prec)
+        h.initFromArray(arr.reshape(-1))
+        gate(h)
+        got = h.getState()
+        del h
+        torch.cuda.empty_cache()
+        return got
+
+    if study.startswith("commutator"):
+        # ||x iQFT(x QFT psi) - iQFT(x QFT(x psi)) - i psi|| (comm_err.py:49-73)
+        dx = math.sqrt(2 * math.pi / N)
+        x = ((np.arange(N) - (N - 1) / 2) * dx)[None, :]
+        qft = lambda a: run([q, q], a, lambda h: h.ftQ2P(1)).reshape(N, N)
+        iqft = lambda a: run([q, q], a, lambda h: h.ftP2Q(1)).reshape(N, N)
+        psi = F.astype(np.complex128)
+        t1 = iqft(x * qft(psi))
+        t2 = iqft(x * qft(x * psi))
+        return np.linalg.norm(x * t1 - t2 - 1j * psi, axis=1)
+    if study == "beam_splitter":
+        # |n>|0> -> sum_k sqrt(C(n,k)) cos^k(t/2) (-i sin(t/2))^(n-k) |k>|n-k>
+        # (beam_splitter_err.py:31-65)
+        th = par["theta"]
+        arr = F[:, :, None] * F[0][None, None, :]
+        got = run([q, q, q], arr.astype(np.complex128), lambda h: h.bs(1, 2, th)).reshape(N, N, N)
+        c, s_ = math.cos(th / 2), math.sin(th / 2)
+        eps = np.empty(N)
+        for n in range(N):
+            k = np.arange(n + 1)
+            lw = 0.5 * (math.lgamma(n + 1) - np.array([math.lgamma(i + 1) + math.lgamma(n - i + 1) for i in k]))
+            with np.errstate(divide="ignore"):
+                lw = lw + k * math.log(abs(c)) + (n - k) * math.log(abs(s_))
+            w = np.exp(lw) * np.sign(c) ** k * ((-1j * np.sign(s_)) ** (n - k))
+            target = (F[k].T * w) @ F[n - k]
+            eps[n] = np.linalg.norm(got[n] - target)
+        return eps
+    if study == "displacement":
+        a = par["alpha"]
+        got = run([q, q], F.astype(np.complex128), lambda h: h.d(1, a)).reshape(N, N)
+        target = _fock_table(N, shift=a * math.sqrt(2.0))
+    elif study == "rotation":
+        th = par["theta"]
+        got = run([q, q], F.astype(np.complex128), lambda h: h.r(1, th)).reshape(N, N)
+        target = F * np.exp(-1j * (np.arange(N) + 0.5) * th)[:, None]
+    else:
+        r = par["r"]
+        got = run([q, q], F.astype(np.complex128), lambda h: h.s(1, r)).reshape(N, N)
+        target = _fock_table(N, scale=math.exp(r), pref=math.exp(r / 2))
+    return np.linalg.norm(got - target, axis=1)
+
+
+def gate_error_envelope(precs=("complex128", "mixed")):
+    """The reference's commutator, displacement D(4), rotation R(pi/4), squeeze
+    S(1) and beam-splitter BS(pi/2) error studies rerun on this library in both
+    precisions: eps_n per Fock state (the commutator's cumulative over n, as
+    there), the upper-bound fit log eps <= a k + b over the points with eps >=
+    1e-9 (complex128) or >= 10x the rounding floor (mixed) up to the first n with
+    eps >= 0.1 (not before n = 4 where the reference waits), then a(N) = c_a N^-1/2
+    + d_a and b(N) = c_b N^1/2 + d_b over the grid sizes."""
+    out = {}
+    for study, par in GATE_STUDIES.items():
+        res = {"per_N": [], "fits": {}}
+        fits = {p: [] for p in precs}
+        for N in par["N"]:
+            row = {"N": N}
+            eps_by = {}
+            for prec in precs:
+                eps = _gate_study_eps(study, N, prec)
+                if study.startswith("commutator"):
+                    eps = np.cumsum(eps)
+                eps_by[prec] = eps
+                over = np.nonzero(eps >= 0.1)[0]
+                over = over[over >= par.get("min_n", 0)]
+                stop = int(over[0]) if len(over) else N - 1
+                floor = float(np.median(eps[:8]))
+                cut = 1e-9 if prec == "complex128" else 10 * floor
+                ks = [k for k in range(stop + 1) if eps[k] >= cut]
+                ab = _upper_fit(ks, eps[ks])
+                row[prec] = {"floor": floor, "n_stop": stop, "fit_points": len(ks),
+                             "a": ab[0] if ab else None, "b": ab[1] if ab else None}
+                if ab:
+                    fits[prec].append((N, ab[0], ab[1]))
+            if len(precs) == 2:
+                stop = row["complex128"]["n_stop"]
+                row["mixed_vs_c128_max"] = float(np.max(np.abs(eps_by["mixed"][:stop + 1] -
+                                                               eps_by["complex128"][:stop + 1])))
+            res["per_N"].append(row)
+            print(f"[bench] {study} N={N}: " + "; ".join(
+                f"{p} stop n={row[p]['n_stop']} floor {row[p]['floor']:.1e}" for p in precs),
+                file=sys.stderr, flush=True)
+        for prec, fp in fits.items():
+            if len(fp) >= 2:
+                Ns = np.array([f[0] for f in fp], dtype=float)
+                ca, da = np.polyfit(Ns ** -0.5, [f[1] for f in fp], 1)
+                cb, db = np.polyfit(Ns ** 0.5, [f[2] for f in fp], 1)
+                res["fits"][prec] = {"a(N)": [float(ca), float(da)], "b(N)": [float(cb), float(db)]}
+        out[study] = res
+    return out
+
+
 def pmc_traffic(kind):
     """(HBM bytes per launch, note) of the FFT pass kind `kind` (fft_row, fft_half,
     fft_fused) from the committed counter summary of the [14,14] complex64
@@ -947,6 +1087,8 @@ def main():
                     help="only the plain-copy bandwidth probe (counter calibration runs)")
     ap.add_argument("--qft-error", action="store_true",
                     help="only the QFT precision envelope (Fock states, both precisions)")
+    ap.add_argument("--gate-error", action="store_true",
+                    help="only the commutator, D, R, S and BS error studies (Fock states, both precisions)")
     ap.add_argument("--no-fuse", action="store_true",
                     help="run every register transform whole (two passes for the strided register)")
     ap.add_argument("--dft-gemm", action="store_true",
@@ -962,6 +1104,9 @@ def main():
         return
     if args.qft_error:
         print(json.dumps({"qft_error": qft_error_envelope()}))
+        return
+    if args.gate_error:
+        print(json.dumps({"gate_error": gate_error_envelope()}))
         return
     if args.copy_only:
         print(json.dumps({"copy": copy_probe()}))

@@ -1087,6 +1087,8 @@ def main():
                     help="only the plain-copy bandwidth probe (counter calibration runs)")
     ap.add_argument("--qft-error", action="store_true",
                     help="only the QFT precision envelope (Fock states, both precisions)")
+    ap.add_argument("--no-errors", action="store_true",
+                    help="skip the precision-envelope fits (QFT and gate error studies)")
     ap.add_argument("--gate-error", action="store_true",
                     help="only the commutator, D, R, S and BS error studies (Fock states, both precisions)")
     ap.add_argument("--no-fuse", action="store_true",
@@ -1182,6 +1184,10 @@ def main():
         out["c5"] = c5_section(max(1, args.steps))
     if not args.no_c4 and not args.only:
         out["c4"] = c4_section(args.c4_circuits, args.c4_seq, max(1, args.steps))
+    if not args.no_errors and not args.only:
+        # the precision envelope (QFT and gate error studies): scaling fits only
+        out["error_fits"] = {"qft": qft_error_envelope()["fits"],
+                             "gates": {k: v["fits"] for k, v in gate_error_envelope().items()}}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ops, args.cpu_seconds)
     print(json.dumps(out))

@@ -719,6 +719,49 @@ def copy_probe(reps=10):
     return {"bytes_per_copy": nbytes, "ms": ms, "gbps": nbytes / (ms * 1e-3) / 1e9, "patterns": patterns}
 
 
+def _bf16(a):
+    """Round float32 values to bfloat16 (nearest, ties to even), returned as float32."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32)
+    u = (u + np.uint32(0x7FFF) + ((u >> np.uint32(16)) & np.uint32(1))) & np.uint32(0xFFFF0000)
+    return u.view(np.float32)
+
+
+def dft_bf16_split_error(Ns=(64, 256, 1024), cols=64, seed=5):
+    """Accuracy of the split-bf16 three-pass DFT-as-GEMM (matrix-core form): F_N
+    and X each split as hi + lo bfloat16 (lo = bf16(v - hi)), Y = F_hi X_hi +
+    F_hi X_lo + F_lo X_hi with the real and imaginary parts as real GEMMs, bf16
+    products exact and summed in FP32 (the MFMA accumulator), against the FP64
+    DFT.  Emulated on the CPU in numpy (the rounding is what the matrix cores
+    would do; the FP32 accumulation order differs).  Returns, per N, the L2
+    relative error over columns of unit-norm complex Gaussian data, to compare
+    with the complex64 FFT's floor (the QFT envelope's 1.0-1.3e-7)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for N in Ns:
+        j = np.arange(N)
+        F = np.exp(-2j * np.pi * np.outer(j, j) / N) / math.sqrt(N)
+        X = rng.standard_normal((N, cols)) + 1j * rng.standard_normal((N, cols))
+        X /= np.linalg.norm(X, axis=0)
+        exact = F @ X
+
+        def split(v):
+            hi = _bf16(v.astype(np.float32))
+            lo = _bf16((v - hi.astype(np.float64)).astype(np.float32))
+            return hi.astype(np.float64), lo.astype(np.float64)
+
+        def gemm3(A, B):  # real 3-pass product, products exact, sum rounded to FP32
+            (ah, al), (bh, bl) = split(A), split(B)
+            return (ah @ bh + ah @ bl + al @ bh).astype(np.float32).astype(np.float64)
+
+        Fr, Fi, Xr, Xi = F.real, F.imag, X.real, X.imag
+        Y = (gemm3(Fr, Xr) - gemm3(Fi, Xi)) + 1j * (gemm3(Fr, Xi) + gemm3(Fi, Xr))
+        err = np.linalg.norm(Y - exact, axis=0) / np.linalg.norm(exact, axis=0)
+        out.append({"N": N, "rel_l2_median": float(np.median(err)), "rel_l2_max": float(np.max(err))})
+        print(f"[bench] split-bf16 DFT N={N}: relative L2 error median {np.median(err):.2e}, "
+              f"max {np.max(err):.2e}", file=sys.stderr, flush=True)
+    return out
+
+
 def dft_gemm_probe(reps=5):
     """DFT-as-GEMM against the register FFT on one 2^28-amplitude complex64 state.
 
@@ -1093,6 +1136,8 @@ def main():
                     help="only the commutator, D, R, S and BS error studies (Fock states, both precisions)")
     ap.add_argument("--no-fuse", action="store_true",
                     help="run every register transform whole (two passes for the strided register)")
+    ap.add_argument("--dft-bf16-error", action="store_true",
+                    help="CPU emulation of the split-bf16 three-pass DFT-as-GEMM's accuracy")
     ap.add_argument("--dft-gemm", action="store_true",
                     help="only the DFT-as-GEMM (MFMA) against register-FFT comparison")
     args = ap.parse_args()
@@ -1101,6 +1146,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if args.dft_bf16_error:
+        print(json.dumps({"dft_bf16_split_error": dft_bf16_split_error()}))
+        return
     if args.dft_gemm:
         print(json.dumps({"dft_gemm": dft_gemm_probe()}))
         return
